@@ -1,0 +1,411 @@
+// capi.hip -- C ABI of the OSEM hot path (include/hrrt_osem.h): context
+// creation (geometry tables -> device), scratch management and the drivers
+// that chain the kernels of kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hrrt_osem.h"
+#include "geometry.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(HRRT_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct hrrt_ctx {
+    hrrt::Geometry geo;
+    hrrt::DevGeom dg{};
+    std::vector<void *> allocs;      // geometry tables
+    // scratch (allocated on first use, zero-initialised once)
+    float *sub_ratio = nullptr;      // [sviews/2][2][N][spp]
+    float *bufA = nullptr, *bufB = nullptr;   // [sviews/2][N][N][Z4]
+    float *pc1 = nullptr;            // [2][N][spp]
+    int *colmask = nullptr;          // [views][N]
+    uint8_t *gapmask = nullptr;      // [th][def_angles][def_elements]
+    float *h_img = nullptr, *h_prj = nullptr;  // device staging for the host shims
+    int device = 0;
+
+    template <typename T>
+    int upload(const std::vector<T> &v, const T *&dst) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * (v.empty() ? 1 : v.size())) != hipSuccess)
+            return fail(HRRT_ERR_NOMEM, "hipMalloc of a geometry table failed");
+        allocs.push_back(p);
+        if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+        dst = static_cast<const T *>(p);
+        return HRRT_OK;
+    }
+    int zalloc(void **p, size_t bytes) {
+        if (*p) return HRRT_OK;
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            return fail(HRRT_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+        }
+        HIP_TRY(hipMemset(*p, 0, bytes));
+        return HRRT_OK;
+    }
+    size_t vol_floats() const { return (size_t)geo.N * geo.N * geo.z4; }
+    size_t col_floats() const { return (size_t)2 * geo.N * geo.spp; }
+    int npairs() const { return geo.sviews / 2; }
+    int ensure_subset_scratch() {
+        int r;
+        if ((r = zalloc((void **)&sub_ratio, sizeof(float) * col_floats() * npairs()))) return r;
+        if ((r = zalloc((void **)&bufA, sizeof(float) * vol_floats() * npairs()))) return r;
+        if ((r = zalloc((void **)&bufB, sizeof(float) * vol_floats() * npairs()))) return r;
+        return HRRT_OK;
+    }
+    int ensure_pair_scratch() {
+        int r;
+        if ((r = zalloc((void **)&pc1, sizeof(float) * col_floats()))) return r;
+        if ((r = zalloc((void **)&bufA, sizeof(float) * vol_floats() * (npairs() > 0 ? npairs() : 1)))) return r;
+        if ((r = zalloc((void **)&bufB, sizeof(float) * vol_floats() * (npairs() > 0 ? npairs() : 1)))) return r;
+        return HRRT_OK;
+    }
+    ~hrrt_ctx() {
+        for (void *p : allocs) (void)hipFree(p);
+        void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, (void *)dg.nzmask};
+        for (void *p : s)
+            if (p) (void)hipFree(p);
+    }
+};
+
+static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+static inline int bits(float f) {
+    int i;
+    std::memcpy(&i, &f, 4);
+    return i;
+}
+
+extern "C" const char *hrrt_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
+    if (!params || !out) return fail(HRRT_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto *ctx = new (std::nothrow) hrrt_ctx();
+    if (!ctx) return fail(HRRT_ERR_NOMEM, "host allocation failed");
+    std::string err;
+    hrrt::Geometry &G = ctx->geo;
+    if (!G.build(*params, err)) {
+        delete ctx;
+        return fail(HRRT_ERR_ARG, "%s", err.c_str());
+    }
+    (void)hipGetDevice(&ctx->device);
+    const int N = G.N, c = N / 2, GM = G.groupmax, ZS = G.zshift, Z = G.z_pixels;
+    hrrt::DevGeom &d = ctx->dg;
+    d.N = N; d.c = c; d.views = G.views; d.Z = Z; d.Z4 = G.z4; d.NQ = G.z4 / 4; d.S = G.segmentsize;
+    d.th = G.th_pixels; d.K = G.K; d.GM = GM; d.ZS = ZS; d.spp = G.spp; d.sviews = G.sviews;
+    d.subsets = G.p.subsets; d.x_rebin = G.x_rebin; d.v_rebin = G.v_rebin; d.def_elements = G.p.def_elements;
+    d.def_angles = G.p.def_angles; d.npskip = G.p.npskip; d.y_size = G.y_size; d.eps1 = G.p.epsilon1;
+    d.eps2 = G.p.epsilon2; d.eps3 = G.p.epsilon3; d.image_max = G.p.image_max;
+
+    // forward-projector interpolation tables (flookup PJ:544-562 and the middle-plane
+    // positions computed inline at PJ:1929-1940)
+    std::vector<int4> mid((size_t)N * GM), endp((size_t)N * GM * 2 * ZS);
+    std::vector<int> skip((size_t)N * GM, 1);
+    for (int y = 1; y < N; y++) {
+        for (int k = 0; k < GM; k++) {
+            const int th = 2 * k + 2, ym = G.yr_bottom[th] + ZS;
+            const float t = G.tantheta[k * Z + ym];
+            float zz = ym + (y - c) * t;
+            int z = (int)zz;
+            float c1 = zz - z;
+            zz = ym + (c - y) * t;
+            int z2 = (int)zz;
+            float c2 = zz - z2;
+            mid[(size_t)y * GM + k] = make_int4(z, z2, bits(c1), bits(c2));
+            const size_t a = ((size_t)y * GM + k) * 2 * ZS, bb = ((size_t)(N - y) * GM + k) * 2 * ZS;
+            skip[(size_t)y * GM + k] = (G.fl_z[a] < 0 || G.fl_z[bb + 2 * ZS - 1] > Z - 1) ? 1 : 0;
+            for (int i = 0; i < 2 * ZS; i++)
+                endp[a + i] = make_int4(G.fl_z[a + i], bits(G.fl_frac[a + i]), G.fl_z[bb + i], bits(G.fl_frac[bb + i]));
+        }
+    }
+    // every sample index the projectors will touch must lie inside the staged
+    // columns: forward z, z+1 in [0, Z] (plane Z is the zero pad), and equal
+    // lengths for the paired back-projection lookups (PJ:2986-2989)
+    for (int y = 1; y < N; y++) {
+        for (int k = 0; k < GM; k++) {
+            const size_t li = (size_t)y * GM + k;
+            const int th = 2 * k + 2;
+            if (!skip[li] && y < c) {
+                const int4 m = mid[li];
+                const int nmid = G.yr_top[th] - ZS - (G.yr_bottom[th] + ZS);
+                int lo = std::min(m.x, m.y), hi = std::max(m.x, m.y) + nmid + 1;
+                for (int i = 0; i < 2 * ZS; i++) {
+                    const int4 e = endp[li * 2 * ZS + i];
+                    lo = std::min(lo, std::min(e.x, e.z));
+                    hi = std::max(hi, std::max(e.x, e.z) + 1);
+                }
+                if (lo < 0 || hi > Z) {
+                    delete ctx;
+                    return fail(HRRT_ERR_ARG, "forward interpolation leaves the axial range (y=%d, group=%d)", y, k + 1);
+                }
+            }
+            const size_t a = li, b2 = (size_t)(N - y) * GM + k;
+            if (G.bl_zea[a] - G.bl_zsa[a] != G.bl_zea[b2] - G.bl_zsa[b2]) {
+                delete ctx;
+                return fail(HRRT_ERR_ARG, "unpaired back-projection lookup lengths (y=%d, group=%d)", y, k + 1);
+            }
+        }
+    }
+    std::vector<int4> blut((size_t)N * GM);
+    for (size_t i = 0; i < blut.size(); i++)
+        blut[i] = make_int4(G.bl_zsa[i], G.bl_zea[i], G.bl_yra[i] - G.bl_zsa[i], bits(G.bl_frac[i]));
+    // rotation sample positions, accumulated along y exactly as rotateimage2 does (PJ:1148-1184)
+    std::vector<float2> rot((size_t)(G.views / 2) * N * N, make_float2(-1.f, -1.f));
+    for (int v = 0; v < G.views / 2; v++) {
+        const float cosv = G.cos_psi[v] / 1.0f, sinv = -G.sin_psi[v] / 1.0f;
+        const float xxx0 = -N / 2 * cosv + N / 2 * sinv + N / 2;
+        const float yyy0 = -N / 2 * sinv - N / 2 * cosv + N / 2;
+        for (int x = 0; x < N; x++) {
+            const int ys = G.cyl_lo[x], ye = G.cyl_hi[x];
+            float xx0 = xxx0 - ys * sinv + (float)(unsigned)x * cosv;
+            float yy0 = yyy0 + ys * cosv + (float)(unsigned)x * sinv;
+            for (int y = ys; y < ye; y++) {
+                rot[((size_t)v * N + x) * N + y] = make_float2(xx0, yy0);
+                xx0 -= sinv;
+                yy0 += cosv;
+            }
+        }
+    }
+    std::vector<int> tasks;
+    for (int k = 0; k <= G.K; k++)
+        for (int yr = G.yr_bottom[2 * k]; yr <= G.yr_top[2 * k]; yr += 4) tasks.push_back((k << 16) | yr);
+    std::vector<int16_t> pseg(G.spp, -1), plane_seg(G.segmentsize, 0);
+    for (int s = 0; s < G.th_pixels; s++)
+        for (int i = 0; i < G.nplanes(s); i++) {
+            pseg[G.seg_data[s] + i] = (int16_t)s;
+            plane_seg[G.segment_offset[s] + i] = (int16_t)s;
+        }
+    std::vector<int> file_seg(G.th_pixels);
+    for (int s = 0; s < G.th_pixels; s++) {
+        const int sw = s % 2 == 1 ? s + 1 : (s == 0 ? 0 : s - 1);   // FIO:116-118
+        file_seg[s] = G.seg_offset[sw] / (N * G.views);
+    }
+    std::vector<float> resc = G.rescalearray;
+
+    int r = 0;
+    if ((r = ctx->upload(G.cyl_lo, d.cyl_lo)) || (r = ctx->upload(G.cyl_hi, d.cyl_hi)) ||
+        (r = ctx->upload(G.sin_psi, d.sin_psi)) || (r = ctx->upload(G.cos_psi, d.cos_psi)) ||
+        (r = ctx->upload(G.yr_bottom, d.yb)) || (r = ctx->upload(G.yr_top, d.yt)) ||
+        (r = ctx->upload(G.seg_data, d.seg_data)) || (r = ctx->upload(G.segment_offset, d.segment_offset)) ||
+        (r = ctx->upload(file_seg, d.file_seg)) || (r = ctx->upload(G.bp_rescale, d.bp_rescale)) ||
+        (r = ctx->upload(resc, d.rescalearray)) || (r = ctx->upload(mid, d.fp_mid)) ||
+        (r = ctx->upload(skip, d.fp_skip)) || (r = ctx->upload(endp, d.fp_end)) || (r = ctx->upload(blut, d.bp_lut)) ||
+        (r = ctx->upload(rot, d.rot)) || (r = ctx->upload(tasks, d.fp_tasks)) || (r = ctx->upload(pseg, d.pseg)) ||
+        (r = ctx->upload(plane_seg, d.plane_seg))) {
+        delete ctx;
+        return r;
+    }
+    d.ntasks = (int)tasks.size();
+    {
+        std::vector<int8_t> nz((size_t)(G.views / 2) * (c + 1), 1);
+        void *p = nullptr;
+        if (hipMalloc(&p, nz.size()) != hipSuccess) {
+            delete ctx;
+            return fail(HRRT_ERR_NOMEM, "hipMalloc failed");
+        }
+        d.nzmask = static_cast<int8_t *>(p);
+        if (hipMemcpy(p, nz.data(), nz.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            delete ctx;
+            return fail(HRRT_ERR_HIP, "hipMemcpy failed");
+        }
+    }
+    *out = ctx;
+    return HRRT_OK;
+}
+
+extern "C" void hrrt_destroy(hrrt_ctx *ctx) { delete ctx; }
+
+extern "C" int hrrt_get_dims(const hrrt_ctx *ctx, hrrt_dims *o) {
+    if (!ctx || !o) return fail(HRRT_ERR_ARG, "null argument");
+    const hrrt::Geometry &G = ctx->geo;
+    o->N = G.N; o->views = G.views; o->z_pixels = G.z_pixels; o->z4 = G.z4; o->segmentsize = G.segmentsize;
+    o->th_pixels = G.th_pixels; o->sviews = G.sviews; o->subsets = G.p.subsets; o->spp = G.spp;
+    o->x_rebin = G.x_rebin; o->v_rebin = G.v_rebin;
+    o->image_floats = (long long)G.N * G.N * G.z4;
+    o->sino_bins = (long long)G.segmentsize * G.views * G.N;
+    o->file_bins = (long long)G.segmentsize * G.p.def_angles * G.p.def_elements;
+    o->paircol_elems = (long long)(G.views / 2) * 2 * G.N * G.spp;
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_get_segments(const hrrt_ctx *ctx, int *yb, int *yt, int *so, int *sd) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    const hrrt::Geometry &G = ctx->geo;
+    const size_t n = sizeof(int) * G.th_pixels;
+    if (yb) std::memcpy(yb, G.yr_bottom.data(), n);
+    if (yt) std::memcpy(yt, G.yr_top.data(), n);
+    if (so) std::memcpy(so, G.segment_offset.data(), n);
+    if (sd) std::memcpy(sd, G.seg_data.data(), n);
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_get_order(const hrrt_ctx *ctx, int *order, int *vieworder) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    if (order) std::memcpy(order, ctx->geo.order.data(), sizeof(int) * ctx->geo.order.size());
+    if (vieworder) std::memcpy(vieworder, ctx->geo.vieworder.data(), sizeof(int) * ctx->geo.vieworder.size());
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_set_norm_zero_mask(hrrt_ctx *ctx, const int8_t *mask) {
+    if (!ctx || !mask) return fail(HRRT_ERR_ARG, "null argument");
+    size_t n = (size_t)(ctx->geo.views / 2) * (ctx->geo.N / 2 + 1);
+    HIP_TRY(hipMemcpy(ctx->dg.nzmask, mask, n, hipMemcpyHostToDevice));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_get_norm_zero_mask(hrrt_ctx *ctx, int8_t *mask) {
+    if (!ctx || !mask) return fail(HRRT_ERR_ARG, "null argument");
+    size_t n = (size_t)(ctx->geo.views / 2) * (ctx->geo.N / 2 + 1);
+    HIP_TRY(hipMemcpy(mask, ctx->dg.nzmask, n, hipMemcpyDeviceToHost));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_stage_sinograms(hrrt_ctx *ctx, const int16_t *d_prompt_file, const float *d_delayed_file,
+                                    const float *d_norm_file, const float *d_scatter_file, const float *d_atten_file,
+                                    int16_t *d_prompt, float *d_add, float *d_invnorm, void *stream) {
+    if (!ctx || !d_prompt || !d_add || !d_invnorm) return fail(HRRT_ERR_ARG, "null argument");
+    const hrrt::Geometry &G = ctx->geo;
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->colmask, sizeof(int) * G.views * G.N))) return r;
+    HIP_TRY(hipMemsetAsync(ctx->colmask, 0, sizeof(int) * G.views * G.N, S(stream)));
+    if (d_norm_file && G.x_rebin * G.v_rebin > 1)
+        if ((r = ctx->zalloc((void **)&ctx->gapmask, (size_t)G.th_pixels * G.p.def_angles * G.p.def_elements))) return r;
+    HIP_TRY(hrrt::launch_stage(ctx->dg, d_prompt_file, d_delayed_file, d_norm_file, d_scatter_file, d_atten_file,
+                               d_prompt, d_add, d_invnorm, ctx->gapmask, ctx->colmask, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_calculate_normfac(hrrt_ctx *ctx, const float *d_invnorm, float *d_normfac, void *stream) {
+    if (!ctx || !d_invnorm || !d_normfac) return fail(HRRT_ERR_ARG, "null argument");
+    int r;
+    if ((r = ctx->ensure_subset_scratch())) return r;
+    const hrrt::Geometry &G = ctx->geo;
+    for (int s = 0; s < G.p.subsets; s++) {
+        hrrt::PairBatch b{s, G.p.subsets, ctx->npairs()};
+        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, d_invnorm, 1, 1, ctx->bufA, ctx->bufB, S(stream)));
+        HIP_TRY(hrrt::launch_rotate_sens(ctx->dg, b, ctx->bufA, ctx->bufB, d_normfac + (size_t)s * ctx->vol_floats(),
+                                         S(stream)));
+    }
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_init_image(hrrt_ctx *ctx, float *d_image, void *stream) {
+    if (!ctx || !d_image) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_init_image(ctx->dg, d_image, ctx->geo.z_min, ctx->geo.z_max, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream) {
+    if (!ctx || !d_image) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_end_iteration(ctx->dg, d_image, ctx->geo.z_min, ctx->geo.z_max, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subset,
+                                const int16_t *d_prompt, const float *d_add, int isubset, void *stream) {
+    if (!ctx || !d_image || !d_normfac_subset || !d_prompt || !d_add) return fail(HRRT_ERR_ARG, "null argument");
+    const hrrt::Geometry &G = ctx->geo;
+    if (isubset < 0 || isubset >= G.p.subsets) return fail(HRRT_ERR_ARG, "subset %d out of range", isubset);
+    int r;
+    if ((r = ctx->ensure_subset_scratch())) return r;
+    hrrt::PairBatch b{isubset, G.p.subsets, ctx->npairs()};
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, d_prompt, d_add, 1, S(stream)));
+    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_forward_proj3d(hrrt_ctx *ctx, const float *d_image, float *d_prj, int view, void *stream) {
+    if (!ctx || !d_image || !d_prj) return fail(HRRT_ERR_ARG, "null argument");
+    if (view < 0 || view >= ctx->geo.views / 2) return fail(HRRT_ERR_ARG, "view %d out of range", view);
+    int r;
+    if ((r = ctx->ensure_pair_scratch())) return r;
+    hrrt::PairBatch b{view, 0, 1};
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->pc1, nullptr, nullptr, 0, S(stream)));
+    HIP_TRY(hrrt::launch_paircol_to_estimate(ctx->dg, view, ctx->pc1, d_prj, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream) {
+    if (!ctx || !d_prj || !d_ima) return fail(HRRT_ERR_ARG, "null argument");
+    if (view < 0 || view >= ctx->geo.views / 2) return fail(HRRT_ERR_ARG, "view %d out of range", view);
+    int r;
+    if ((r = ctx->ensure_pair_scratch())) return r;
+    hrrt::PairBatch b{view, 0, 1};
+    HIP_TRY(hrrt::launch_estimate_to_paircol(ctx->dg, d_prj, ctx->pc1, S(stream)));
+    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->pc1, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    HIP_TRY(hrrt::launch_rotate_add(ctx->dg, view, ctx->bufA, ctx->bufB, d_ima, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float *d_add, const int16_t *d_prompt,
+                                       void *stream) {
+    if (!ctx || !d_est || !d_add || !d_prompt) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_ratio_w3(ctx->dg, d_est, d_add, d_prompt, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_cal_updateimage(hrrt_ctx *ctx, float *d_image, float *d_correction, const float *d_normfac,
+                                    void *stream) {
+    if (!ctx || !d_image || !d_correction || !d_normfac) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_update_image(ctx->dg, d_image, d_correction, d_normfac, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_normfac_calculate(hrrt_ctx *ctx, const float *d_sens, float *d_out, void *stream) {
+    if (!ctx || !d_sens || !d_out) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_normfac_invert(ctx->dg, d_sens, d_out, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_forward_proj3d_host(hrrt_ctx *ctx, const float *image, float *prj, int view) {
+    if (!ctx || !image || !prj) return fail(HRRT_ERR_ARG, "null argument");
+    int r;
+    const size_t ni = ctx->vol_floats() * sizeof(float);
+    const size_t np = (size_t)2 * ctx->geo.segmentsize * ctx->geo.N * sizeof(float);
+    if ((r = ctx->zalloc((void **)&ctx->h_img, ni)) || (r = ctx->zalloc((void **)&ctx->h_prj, np))) return r;
+    HIP_TRY(hipMemcpy(ctx->h_img, image, ni, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->h_prj, prj, np, hipMemcpyHostToDevice));
+    if ((r = hrrt_forward_proj3d(ctx, ctx->h_img, ctx->h_prj, view, nullptr))) return r;
+    HIP_TRY(hipMemcpy(prj, ctx->h_prj, np, hipMemcpyDeviceToHost));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_back_proj3d_host(hrrt_ctx *ctx, float *prj, float *ima, int view) {
+    if (!ctx || !prj || !ima) return fail(HRRT_ERR_ARG, "null argument");
+    int r;
+    const size_t ni = ctx->vol_floats() * sizeof(float);
+    const size_t np = (size_t)2 * ctx->geo.segmentsize * ctx->geo.N * sizeof(float);
+    if ((r = ctx->zalloc((void **)&ctx->h_img, ni)) || (r = ctx->zalloc((void **)&ctx->h_prj, np))) return r;
+    HIP_TRY(hipMemcpy(ctx->h_img, ima, ni, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->h_prj, prj, np, hipMemcpyHostToDevice));
+    if ((r = hrrt_back_proj3d(ctx, ctx->h_prj, ctx->h_img, view, nullptr))) return r;
+    HIP_TRY(hipMemcpy(prj, ctx->h_prj, np, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ima, ctx->h_img, ni, hipMemcpyDeviceToHost));
+    return HRRT_OK;
+}

@@ -1,0 +1,61 @@
+// kernels.h -- device-side tables and kernel launch interface (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hrrt {
+
+// Geometry tables resident on the device (uploaded once per context).
+struct DevGeom {
+    int N, c, views, Z, Z4, NQ, S, th, K, GM, ZS, spp, sviews, subsets;
+    int x_rebin, v_rebin, def_elements, def_angles, npskip;
+    float y_size, eps1, eps2, eps3, image_max;
+    const int *cyl_lo, *cyl_hi;          // [N]
+    const float *sin_psi, *cos_psi;      // [views]
+    const int *yb, *yt;                  // [th] plane ranges
+    const int *seg_data;                 // [th] padded pair-column start
+    const int *segment_offset;           // [th] plane offsets in [S]
+    const int *file_seg;                 // [th] file plane offset (swapped segment, FIO:116) in file planes
+    const float *bp_rescale;             // [th]
+    const float *rescalearray;           // [GM][Z]
+    const int4 *fp_mid;                  // [N][GM]: {z, z2, bits(c1), bits(c2)}
+    const int *fp_skip;                  // [N][GM]
+    const int4 *fp_end;                  // [N][GM][2ZS]: {z, bits(f1), z2, bits(f2)}
+    const int4 *bp_lut;                  // [N][GM]: {zsa, zea, off = yra - zsa, bits(frac)}
+    const float2 *rot;                   // [views/2][N][N] rotation sample positions (rotateimage2 order)
+    const int *fp_tasks;                 // [ntasks]: (k << 16) | yr0
+    int ntasks;
+    const int16_t *pseg;                 // [spp] padded plane -> segment (-1 on guards)
+    const int16_t *plane_seg;            // [S] internal plane -> segment
+    int8_t *nzmask;                      // [views/2][N/2+1], 0 = masked column
+};
+
+// One batch of view pairs processed together (a subset, or a single pair).
+struct PairBatch {
+    int view0, view_stride, npairs;      // view of local pair lp = view0 + view_stride*lp
+};
+
+hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *image, float *out_paircol,
+                          const int16_t *prompt, const float *add, int ratio_mode, hipStream_t s);
+hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const float *src_paircol, int src_global,
+                              int apply_rescale, float *bufA, float *bufB, hipStream_t s);
+hipError_t launch_rotate_update(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
+                                float *image, const float *normfac, hipStream_t s);
+hipError_t launch_rotate_sens(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
+                              float *normfac_out, hipStream_t s);
+hipError_t launch_rotate_add(const DevGeom &g, int view, const float *bufA, const float *bufB, float *ima,
+                             hipStream_t s);
+hipError_t launch_paircol_to_estimate(const DevGeom &g, int view, const float *paircol, float *prj, hipStream_t s);
+hipError_t launch_estimate_to_paircol(const DevGeom &g, float *prj, float *paircol, hipStream_t s);
+hipError_t launch_ratio_w3(const DevGeom &g, float *est, const float *add, const int16_t *prompt, hipStream_t s);
+hipError_t launch_update_image(const DevGeom &g, float *image, float *corr, const float *nf, hipStream_t s);
+hipError_t launch_normfac_invert(const DevGeom &g, const float *sens, float *out, hipStream_t s);
+hipError_t launch_init_image(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s);
+hipError_t launch_end_iteration(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s);
+hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *delayed_f, const float *norm_f,
+                        const float *scatter_f, const float *atten_f, int16_t *prompt, float *add, float *invnorm,
+                        uint8_t *gapmask, int *colmask, hipStream_t s);
+
+}  // namespace hrrt

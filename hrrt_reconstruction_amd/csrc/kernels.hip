@@ -1,0 +1,757 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the OSEM hot path.
+//
+// Every kernel issues its float operations in the order the reference does
+// (no fused multiply-add: this file is compiled with -ffp-contract=off and
+// the pragma below), so results are bit-identical to the reference
+// arithmetic for the same inputs.  Citations: PJ = hrrt_osem3d/
+// hrrt_osem3dv_sbrt_span3.cpp, S3 = hrrt_osem3d/hrrt_osem3d_span3.cpp,
+// FIO = hrrt_osem3d/file_io_processor.cpp.
+#include "kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace hrrt {
+
+namespace {
+
+__device__ __forceinline__ float4 f4(float a) { return make_float4(a, a, a, a); }
+__device__ __forceinline__ float4 add4(float4 a, float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float4 sub4(float4 a, float4 b) {
+    return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+}
+__device__ __forceinline__ float4 mul4(float4 a, float s) {
+    return make_float4(a.x * s, a.y * s, a.z * s, a.w * s);
+}
+__device__ __forceinline__ float lane(const float4 &v, int l) {
+    return l == 0 ? v.x : (l == 1 ? v.y : (l == 2 ? v.z : v.w));
+}
+
+// Bilinear sample of the z-column chunk q (4 planes) at the rotated position of
+// (xx, yy) (PJ:1865-1892).  Returns false when the reference rejects the sample.
+__device__ __forceinline__ bool sample_chunk(const DevGeom &g, const float4 *__restrict__ image, float cosv,
+                                             float sinv, int xx, int yy, int q, float4 &out) {
+    const int c = g.c, N = g.N;
+    float xx0 = (xx - c) * cosv - (yy - c) * sinv + c;
+    float yy0 = (xx - c) * sinv + (yy - c) * cosv + c;
+    if (xx0 < 0 || xx0 > N - 2 || yy0 < 0 || xx0 > N - 2) return false;
+    int xi = (int)xx0, yi = (int)yy0;
+    float x1 = xx0 - xi, y1 = yy0 - yi;
+    float w1 = (1 - x1) * (1 - y1), w2 = x1 * (1 - y1), w3 = y1 * (1 - x1), w4 = y1 * x1;
+    const size_t NQ = g.NQ;
+    float4 r1 = image[((size_t)xi * N + yi) * NQ + q];
+    float4 r2 = image[((size_t)(xi + 1) * N + yi) * NQ + q];
+    float4 r3 = image[((size_t)xi * N + yi + 1) * NQ + q];
+    float4 r4 = image[((size_t)(xi + 1) * N + yi + 1) * NQ + q];
+    out = add4(add4(mul4(r1, w1), mul4(r2, w2)), add4(mul4(r3, w3), mul4(r4, w4)));
+    return true;
+}
+
+__device__ __forceinline__ void put_lanes(float4 *buf, int q, int lanei, float4 v, int Z) {
+    float *f = reinterpret_cast<float *>(buf);
+    int z = 4 * q;
+    if (z + 0 < Z) f[(z + 0) * 4 + lanei] = v.x;
+    if (z + 1 < Z) f[(z + 1) * 4 + lanei] = v.y;
+    if (z + 2 < Z) f[(z + 2) * 4 + lanei] = v.z;
+    if (z + 3 < Z) f[(z + 3) * 4 + lanei] = v.w;
+}
+
+// The eight symmetric sample positions of (x, y) (PJ:1856-1864).  Samples 0..3
+// feed "pam" (near half of the ray), 4..7 feed "pam2" (mirrored half).
+__device__ __forceinline__ void sym8(int i, int x, int y, int N, int &xx, int &yy) {
+    switch (i) {
+        case 0: xx = x; yy = y; break;
+        case 1: xx = N - x; yy = y; break;
+        case 2: xx = N - y; yy = x; break;
+        case 3: xx = N - y; yy = N - x; break;
+        case 4: xx = x; yy = N - y; break;
+        case 5: xx = N - x; yy = N - y; break;
+        case 6: xx = y; yy = x; break;
+        default: xx = y; yy = N - x; break;
+    }
+}
+
+__device__ __forceinline__ void sample8(const DevGeom &g, const float4 *__restrict__ image, float cosv, float sinv,
+                                        int x, int y, float4 *pam, float4 *pam2) {
+    const int NQ = g.NQ, N = g.N;
+    for (int it = threadIdx.x; it < 8 * NQ; it += blockDim.x) {
+        int i = it / NQ, q = it - i * NQ;
+        int xx, yy;
+        sym8(i, x, y, N, xx, yy);
+        float4 v;
+        if (!sample_chunk(g, image, cosv, sinv, xx, yy, q, v)) v = f4(0.f);
+        put_lanes(i < 4 ? pam : pam2, q, i & 3, v, g.Z);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Forward projector (forward_proj3d_view1_thread PJ:1728-1978 + scatter
+// PJ:1397-1436), optionally fused with the OP ratio (pt_update_est_w3
+// S3:2493-2505) and the back-projection rescale (PJ:1638-1649).
+//
+// One workgroup per (view pair, x in [1, N/2]).  The eight rotated z-columns
+// of each y are staged in LDS (double buffered); every thread owns up to MAXT
+// tasks of 4 consecutive planes of one segment pair and accumulates them in
+// registers over y in the reference's order.
+// ---------------------------------------------------------------------------
+template <int MAXT, bool RATIO>
+__global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatch b,
+                                                 const float4 *__restrict__ image, float *__restrict__ out,
+                                                 const int16_t *__restrict__ prompt, const float *__restrict__ add) {
+    extern __shared__ float4 lds[];
+    const int PL = g.Z4 + 4;
+    float4 *rimc = lds;
+    float4 *stage = lds + PL;  // [2][2][PL]
+    const int x = blockIdx.x + 1, lp = blockIdx.y;
+    const int view = b.view0 + b.view_stride * lp;
+    const int N = g.N, c = g.c, GM = g.GM, ZS = g.ZS;
+    if (g.cyl_lo[x] == g.cyl_hi[x]) return;
+    if (g.nzmask[view * (c + 1) + x] == 0) return;
+    const float cosv = g.cos_psi[view], sinv = g.sin_psi[view];
+
+    for (int i = threadIdx.x; i < 5 * PL; i += blockDim.x) lds[i] = f4(0.f);
+    __syncthreads();
+    // centre row samples (x,c),(N-x,c),(c,x),(c,N-x) PJ:1815-1829
+    for (int it = threadIdx.x; it < 4 * g.NQ; it += blockDim.x) {
+        int i = it / g.NQ, q = it - i * g.NQ;
+        int xx = i == 0 ? x : (i == 1 ? N - x : c);
+        int yy = i < 2 ? c : (i == 2 ? x : N - x);
+        float4 v;
+        if (!sample_chunk(g, image, cosv, sinv, xx, yy, q, v)) v = f4(0.f);
+        put_lanes(rimc, q, i, v, g.Z);
+    }
+    const int ystart = g.cyl_lo[x];
+    if (ystart < c) sample8(g, image, cosv, sinv, x, ystart, stage, stage + PL);
+    __syncthreads();
+
+    float4 acc1[MAXT][4], acc2[MAXT][4];
+    int tk[MAXT], ty0[MAXT], tcnt[MAXT];
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        int task = threadIdx.x + t * blockDim.x;
+        tcnt[t] = 0; tk[t] = 0; ty0[t] = 0;
+        if (task < g.ntasks) {
+            int code = g.fp_tasks[task];
+            int k = code >> 16, yr0 = code & 0xffff;
+            int yt = g.yt[2 * k];
+            tk[t] = k; ty0[t] = yr0;
+            tcnt[t] = min(4, yt - yr0 + 1);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            float4 v = r < tcnt[t] ? rimc[ty0[t] + r] : f4(0.f);
+            acc1[t][r] = v;
+            acc2[t][r] = v;
+        }
+    }
+
+    for (int y = ystart; y < c; y++) {
+        const int st = (y - ystart) & 1;
+        if (y + 1 < c) sample8(g, image, cosv, sinv, x, y + 1, stage + (st ^ 1) * 2 * PL, stage + (st ^ 1) * 2 * PL + PL);
+        const float4 *pa = stage + st * 2 * PL, *pb = pa + PL;
+#pragma unroll
+        for (int t = 0; t < MAXT; t++) {
+            const int cnt = tcnt[t], k = tk[t], yr0 = ty0[t];
+            if (cnt == 0) continue;
+            if (k == 0) {   // theta = 0 PJ:1907-1910
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (r < cnt) acc1[t][r] = add4(acc1[t][r], add4(pa[yr0 + r], pb[yr0 + r]));
+                continue;
+            }
+            const int li = y * GM + k - 1;
+            if (g.fp_skip[li]) continue;   // PJ:1920-1921
+            const int yb = g.yb[2 * k], yt = g.yt[2 * k];
+            const int4 mid = g.fp_mid[li];
+            const float c1 = __int_as_float(mid.z), c2 = __int_as_float(mid.w);
+            if (yr0 >= yb + ZS && yr0 + cnt - 1 <= yt - ZS) {
+                // middle planes PJ:1934-1944: constant fractions, z and z2 advance with yr
+                const int z0 = mid.x + (yr0 - (yb + ZS)), z20 = mid.y + (yr0 - (yb + ZS));
+                // sliding window over z: each plane reuses the upper samples of the previous one
+                float4 a0 = pa[z0], b0 = pb[z0], e0 = pa[z20], d0 = pb[z20];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (r < cnt) {
+                        const float4 a1 = pa[z0 + r + 1], b1 = pb[z0 + r + 1];
+                        const float4 e1 = pa[z20 + r + 1], d1 = pb[z20 + r + 1];
+                        acc1[t][r] = add4(acc1[t][r], add4(mul4(add4(d1, a0), c2), mul4(add4(a1, d0), c1)));
+                        acc2[t][r] = add4(acc2[t][r], add4(mul4(add4(e1, b0), c2), mul4(add4(b1, e0), c1)));
+                        a0 = a1; b0 = b1; e0 = e1; d0 = d1;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    if (r >= cnt) continue;
+                    const int yr = yr0 + r, i = yr - yb;
+                    int z, z2;
+                    float f1, f2;
+                    if (i < ZS || yr > yt - ZS) {   // end planes PJ:1922-1928, 1945-1950
+                        const int e = i < ZS ? i : ZS + (yr - (yt - ZS + 1));
+                        const int4 q = g.fp_end[(size_t)li * 2 * ZS + e];
+                        z = q.x; f1 = __int_as_float(q.y); z2 = q.z; f2 = __int_as_float(q.w);
+                    } else {
+                        z = mid.x + (yr - (yb + ZS)); z2 = mid.y + (yr - (yb + ZS)); f1 = c1; f2 = c2;
+                    }
+                    const float4 a0 = pa[z], a1 = pa[z + 1], b0 = pb[z], b1 = pb[z + 1];
+                    const float4 d0 = pb[z2], d1 = pb[z2 + 1], e0 = pa[z2], e1 = pa[z2 + 1];
+                    acc1[t][r] = add4(acc1[t][r], add4(mul4(add4(d1, a0), f2), mul4(add4(a1, d0), f1)));
+                    acc2[t][r] = add4(acc2[t][r], add4(mul4(add4(e1, b0), f2), mul4(add4(b1, e0), f1)));
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // scaling PJ:1953-1975 and write-out (all loops unrolled: accumulators stay in registers)
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        const int cnt = tcnt[t], k = tk[t], yr0 = ty0[t];
+        if (cnt == 0) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (r >= cnt) continue;
+            if (k == 0) {
+                acc1[t][r] = mul4(acc1[t][r], g.y_size);
+            } else {
+                const float s = g.rescalearray[(k - 1) * g.Z + yr0 + r];
+                acc1[t][r] = mul4(acc1[t][r], s);
+                acc2[t][r] = mul4(acc2[t][r], s);
+            }
+        }
+#pragma unroll
+        for (int si = 0; si < 2; si++) {
+            if (si == 1 && k == 0) continue;
+            const int seg = k == 0 ? 0 : (si == 0 ? 2 * k : 2 * k - 1);
+            const int pidx = g.seg_data[seg] + (yr0 - g.yb[seg]);
+            const float resc = g.bp_rescale[seg];
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                if ((l & 1) && x == c) continue;   // at x == N/2 lanes 1/3 duplicate lanes 0/2
+                const int col = (l & 1) ? N - x : x, h = l >> 1;
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = lane(si == 0 ? acc1[t][r] : acc2[t][r], l);
+                if (RATIO) {
+                    const size_t gi = ((size_t)(view * 2 + h) * N + col) * g.spp + pidx;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        if (r >= cnt) continue;
+                        const float den = v[r] + add[gi + r];
+                        const int16_t pr = prompt[gi + r];
+                        float q;
+                        if (pr == 0) q = 0.0f;
+                        else if (den > g.eps3) q = (float)pr / den;
+                        else q = 1.0f;
+                        v[r] = q * resc;
+                    }
+                }
+                float *o = out + ((size_t)(lp * 2 + h) * N + col) * g.spp + pidx;
+                if (cnt == 4) {
+                    *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                        if (r < cnt) o[r] = v[r];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Back projector, direct variant (back_proj3d2 PJ:1457-1620 with
+// back_proj3d_view_n1 PJ:2927-3086), as a gather.  For a rotated-frame voxel
+// (X, Y) the reference's pass over view v gives A(X,Y) from sinogram column X
+// and its pass over view v+90 gives B(X,Y) from column Y; R = A + B.  One
+// workgroup per (column, role, view pair) stages the padded column in LDS and
+// evaluates every voxel of the disk row/column that reads it.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) bp_kernel(const DevGeom g, const PairBatch b, const float *__restrict__ src,
+                                                 int src_global, int apply_rescale, const int16_t *__restrict__ pseg,
+                                                 float *__restrict__ bufA, float *__restrict__ bufB) {
+    extern __shared__ float colv[];
+    const int col = blockIdx.x + 1, role = blockIdx.y, lp = blockIdx.z;
+    const int view = b.view0 + b.view_stride * lp;
+    const int N = g.N, c = g.c, Z = g.Z, spp = g.spp, GM = g.GM;
+    const int lo = g.cyl_lo[col], hi = g.cyl_hi[col];
+    if (lo == hi) return;
+    const int gp = src_global ? view : lp;
+    const float *cp = src + ((size_t)(gp * 2 + role) * N + col) * spp;
+    for (int i = threadIdx.x; i < spp; i += blockDim.x) {
+        float v = cp[i];
+        if (apply_rescale) {
+            int s = pseg[i];
+            v = s >= 0 ? v * g.bp_rescale[s] : 0.0f;
+        }
+        colv[i] = v;
+    }
+    __syncthreads();
+    const int mcol = col <= c ? col : N - col;
+    const bool masked = g.nzmask[view * (c + 1) + mcol] == 0;
+    float *dst = role == 0 ? bufA : bufB;
+    dst += (size_t)lp * N * N * g.Z4;
+    const int wave = threadIdx.x >> 6, lanei = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int V = lo + wave; V < hi; V += nw) {
+        const int X = role == 0 ? col : V, Y = role == 0 ? V : col;
+        float *o = dst + ((size_t)X * N + Y) * g.Z4;
+        if (masked && V != c) {
+            for (int z = lanei; z < Z; z += 64) o[z] = 0.0f;
+            continue;
+        }
+        const int sl = V <= c ? V : N - V, ll = N - sl;
+        const bool first_even = role == 0 ? (V <= c) : (V >= c);
+        for (int z = lanei; z < Z; z += 64) {
+            const float t0 = colv[g.seg_data[0] + z];
+            float i1 = first_even ? t0 : 0.0f;
+            float i2 = first_even ? 0.0f : t0;
+            for (int k = 1; k <= g.K; k++) {
+                const int4 L1 = g.bp_lut[sl * GM + k - 1];
+                const int4 L2 = g.bp_lut[ll * GM + k - 1];
+                const int yb = g.yb[2 * k];
+                const int be = g.seg_data[2 * k] - yb, bo = g.seg_data[2 * k - 1] - yb;
+                const int b1 = first_even ? be : bo, b2 = first_even ? bo : be;
+                if (z >= L1.x && z <= L1.y) {
+                    const int yr = z + L1.z;
+                    const float p = colv[b1 + yr], p1 = colv[b1 + yr + 1];
+                    i1 = i1 + (p + __int_as_float(L1.w) * (p1 - p));
+                }
+                if (z >= L2.x && z <= L2.x + (L1.y - L1.x)) {
+                    const int yr = z + L2.z;
+                    const float p = colv[b2 + yr], p1 = colv[b2 + yr + 1];
+                    i2 = i2 + (p + __int_as_float(L2.w) * (p1 - p));
+                }
+            }
+            o[z] = i1 + i2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rotation (rotateimage2 PJ:1137-1186) of R = A + B of every pair of the
+// batch, accumulated in pair order, fused with the image update
+// (pt_cal_updateimage S3:2033-2056).  Block (64, 4): x lanes over z chunks of 4.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 rotate_accumulate(const DevGeom &g, const PairBatch &b,
+                                                    const float4 *__restrict__ A, const float4 *__restrict__ B,
+                                                    int x, int y, int q) {
+    const int N = g.N, NQ = g.NQ;
+    float4 corr = f4(0.f);
+    for (int lp = 0; lp < b.npairs; lp++) {
+        const int view = b.view0 + b.view_stride * lp;
+        const float2 pnt = g.rot[((size_t)view * N + x) * N + y];
+        const int xi = (int)pnt.x, yi = (int)pnt.y;
+        if (xi < 0 || xi > N - 2 || yi < 0 || yi > N - 2) continue;
+        const float x1 = pnt.x - xi, y1 = pnt.y - yi;
+        const float w1 = (1 - x1) * (1 - y1), w2 = x1 * (1 - y1), w3 = y1 * (1 - x1), w4 = y1 * x1;
+        const size_t base = (size_t)lp * N * N * NQ + q;
+        const size_t i1 = base + ((size_t)xi * N + yi) * NQ, i2 = base + ((size_t)(xi + 1) * N + yi) * NQ;
+        const size_t i3 = base + ((size_t)xi * N + yi + 1) * NQ, i4 = base + ((size_t)(xi + 1) * N + yi + 1) * NQ;
+        const float4 m1 = add4(A[i1], B[i1]), m2 = add4(A[i2], B[i2]);
+        const float4 m3 = add4(A[i3], B[i3]), m4 = add4(A[i4], B[i4]);
+        corr = add4(corr, add4(add4(mul4(m1, w1), mul4(m2, w2)), add4(mul4(m3, w3), mul4(m4, w4))));
+    }
+    return corr;
+}
+
+__device__ __forceinline__ float update1(float img, float corr, float nf, float image_max) {
+    const float amin = 0.01f, amax = 100.00f;
+    float cl = amin > corr ? amin : corr;   // _mm_max_ps(amin, corr)
+    cl = amax < cl ? amax : cl;             // _mm_min_ps(amax, .)
+    const float t = img * (cl * nf);
+    return image_max < t ? image_max : t;   // _mm_min_ps(image_max, .)
+}
+
+__global__ void __launch_bounds__(256) rot_update_kernel(const DevGeom g, const PairBatch b,
+                                                         const float4 *__restrict__ A, const float4 *__restrict__ B,
+                                                         float4 *__restrict__ image, const float4 *__restrict__ nf) {
+    const int q = threadIdx.x, x = blockIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
+    const float4 corr = rotate_accumulate(g, b, A, B, x, y, q);
+    const size_t i = ((size_t)x * g.N + y) * g.NQ + q;
+    const float4 im = image[i], n = nf[i];
+    image[i] = make_float4(update1(im.x, corr.x, n.x, g.image_max), update1(im.y, corr.y, n.y, g.image_max),
+                           update1(im.z, corr.z, n.z, g.image_max), update1(im.w, corr.w, n.w, g.image_max));
+}
+
+__device__ __forceinline__ float invert1(float s, float eps1) { return fabsf(s) > eps1 ? (1.0f / s) : 0.0f; }
+
+__global__ void __launch_bounds__(256) rot_sens_kernel(const DevGeom g, const PairBatch b,
+                                                       const float4 *__restrict__ A, const float4 *__restrict__ B,
+                                                       float4 *__restrict__ nf) {
+    const int q = threadIdx.x, x = blockIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
+    const float4 s = rotate_accumulate(g, b, A, B, x, y, q);
+    const int z = 4 * q;
+    float4 o;
+    o.x = z + 0 < g.Z ? invert1(s.x, g.eps1) : 0.0f;
+    o.y = z + 1 < g.Z ? invert1(s.y, g.eps1) : 0.0f;
+    o.z = z + 2 < g.Z ? invert1(s.z, g.eps1) : 0.0f;
+    o.w = z + 3 < g.Z ? invert1(s.w, g.eps1) : 0.0f;
+    nf[((size_t)x * g.N + y) * g.NQ + q] = o;
+}
+
+__global__ void __launch_bounds__(256) rot_add_kernel(const DevGeom g, int view, const float4 *__restrict__ A,
+                                                      const float4 *__restrict__ B, float4 *__restrict__ ima) {
+    const int q = threadIdx.x, x = blockIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
+    PairBatch b{view, 0, 1};
+    const float4 r = rotate_accumulate(g, b, A, B, x, y, q);
+    const size_t i = ((size_t)x * g.N + y) * g.NQ + q;
+    ima[i] = add4(ima[i], r);
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversions for the per-view-pair API.
+// ---------------------------------------------------------------------------
+// pair-column -> estimate [2S][N], covered bins only (PJ:1397-1436).
+__global__ void paircol_to_estimate_kernel(const DevGeom g, int view, const float *__restrict__ pc,
+                                           float *__restrict__ prj) {
+    const int col = blockIdx.x, h = blockIdx.y;
+    const int N = g.N, c = g.c;
+    if (col == 0) return;
+    const int m = col <= c ? col : N - col;
+    if (g.cyl_lo[m] == 0 || g.nzmask[view * (c + 1) + m] == 0) return;
+    const float *src = pc + ((size_t)h * N + col) * g.spp;
+    for (int p = threadIdx.x; p < g.S; p += blockDim.x) {
+        int s = 0;
+        while (s + 1 < g.th && g.segment_offset[s + 1] <= p) s++;
+        const int i = p - g.segment_offset[s];
+        prj[((size_t)h * g.S + p) * N + col] = src[g.seg_data[s] + i];
+    }
+}
+
+// estimate [2S][N] -> pair-column with the back-projection rescale; the
+// estimate itself is rescaled in place as back_proj3d2 does (PJ:1496-1507).
+__global__ void estimate_to_paircol_kernel(const DevGeom g, float *__restrict__ prj, float *__restrict__ pc) {
+    const int p = blockIdx.x, h = blockIdx.y;
+    int s = 0;
+    while (s + 1 < g.th && g.segment_offset[s + 1] <= p) s++;
+    const int i = p - g.segment_offset[s];
+    const float r = g.bp_rescale[s];
+    for (int col = threadIdx.x; col < g.N; col += blockDim.x) {
+        float *e = prj + ((size_t)h * g.S + p) * g.N + col;
+        const float v = *e * r;
+        *e = v;
+        pc[((size_t)h * g.N + col) * g.spp + g.seg_data[s] + i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels.
+// ---------------------------------------------------------------------------
+__global__ void ratio_w3_kernel(const DevGeom g, float *__restrict__ est, const float *__restrict__ add,
+                                const int16_t *__restrict__ prompt, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float den = est[i] + add[i];
+        const int16_t pr = prompt[i];
+        if (pr == 0) est[i] = 0.0f;
+        else if (den > g.eps3) est[i] = (float)pr / den;
+        else est[i] = 1.0f;
+    }
+}
+
+__global__ void update_image_kernel(const DevGeom g, float *__restrict__ image, float *__restrict__ corr,
+                                    const float *__restrict__ nf) {
+    const int x = blockIdx.x;
+    for (int y = g.cyl_lo[x] + threadIdx.y; y < g.cyl_hi[x]; y += blockDim.y)
+        for (int z = threadIdx.x; z < g.Z4; z += blockDim.x) {
+            const size_t i = ((size_t)x * g.N + y) * g.Z4 + z;
+            image[i] = update1(image[i], corr[i], nf[i], g.image_max);
+            corr[i] = 0.0f;
+        }
+}
+
+__global__ void normfac_invert_kernel(const DevGeom g, const float *__restrict__ sens, float *__restrict__ out) {
+    const int x = blockIdx.x;
+    for (int y = g.cyl_lo[x] + threadIdx.y; y < g.cyl_hi[x]; y += blockDim.y)
+        for (int z = threadIdx.x; z < g.Z; z += blockDim.x) {
+            const size_t i = ((size_t)x * g.N + y) * g.Z4 + z;
+            out[i] = invert1(sens[i], g.eps1);
+        }
+}
+
+__global__ void init_image_kernel(const DevGeom g, float *__restrict__ image, int z_min, int z_max) {
+    const int x = blockIdx.x;
+    for (int y = threadIdx.y; y < g.N; y += blockDim.y)
+        for (int z = threadIdx.x; z < g.Z4; z += blockDim.x) {
+            const bool in = y >= g.cyl_lo[x] && y < g.cyl_hi[x] && z >= z_min && z < z_max;
+            image[((size_t)x * g.N + y) * g.Z4 + z] = in ? 1.0f : 0.0f;
+        }
+}
+
+__global__ void end_iteration_kernel(const DevGeom g, float *__restrict__ image, int z_min, int z_max) {
+    const int x = blockIdx.x;
+    for (int y = g.cyl_lo[x] + threadIdx.y; y < g.cyl_hi[x]; y += blockDim.y)
+        for (int z = z_min + (int)threadIdx.x; z < z_max; z += blockDim.x) {
+            float *v = image + ((size_t)x * g.N + y) * g.Z4 + z;
+            if ((double)*v < 1e-16) *v = 0.0f;
+        }
+}
+
+__global__ void fill_kernel(float4 *__restrict__ p, size_t n4) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Staging (prepare_osem3dw3 S3:1666-1770, prepare_norm S3:841-1003 with
+// norm_ANW S3:796-821, read_flat_* FIO:110-259, sino_rebin FIO:29-74).
+// ---------------------------------------------------------------------------
+struct StageArgs {
+    const int16_t *prompt_f;
+    const float *delayed_f, *norm_f, *scatter_f, *atten_f;
+    int16_t *prompt;
+    float *add, *invnorm;
+    const uint8_t *gapmask;   // [th][def_angles][def_elements] when rebinning the norm
+    int *colmask;             // [views][N]
+    const int16_t *plane_seg; // [S]
+};
+
+__device__ __forceinline__ float rebin_sum(const DevGeom &g, const float *f, size_t fplane, int v, int col) {
+    const int DE = g.def_elements;
+    if (g.x_rebin * g.v_rebin == 1) return f[fplane * (size_t)g.def_angles * DE + (size_t)v * DE + col];
+    // sino_rebin FIO:38-66: views summed first (in place), then radial bins
+    const float *p = f + fplane * (size_t)g.def_angles * DE;
+    float out = 0.0f;
+    for (int xr = 0; xr < g.x_rebin; xr++) {
+        float s = p[(size_t)(v * g.v_rebin) * DE + col * g.x_rebin + xr];
+        for (int vr = 1; vr < g.v_rebin; vr++) s += p[(size_t)(v * g.v_rebin + vr) * DE + col * g.x_rebin + xr];
+        out = xr == 0 ? s : out + s;
+    }
+    return out;
+}
+
+__device__ __forceinline__ float rebin_norm(const DevGeom &g, const float *f, size_t fplane, int v, int col) {
+    const int DE = g.def_elements;
+    const float *p = f + fplane * (size_t)g.def_angles * DE;
+    float out = 0.0f;
+    for (int xr = 0; xr < g.x_rebin; xr++) {
+        float s = p[(size_t)(v * g.v_rebin) * DE + col * g.x_rebin + xr];
+        if (s > 0.0f) s = 1.0f / s;
+        for (int vr = 1; vr < g.v_rebin; vr++) {
+            const float a = p[(size_t)(v * g.v_rebin + vr) * DE + col * g.x_rebin + xr];
+            if (a > 0.0f) s += 1.0f / a;
+        }
+        out = xr == 0 ? s : out + s;
+    }
+    if (out > 0.0f) out = 1.0f / out;
+    return out;
+}
+
+__device__ __forceinline__ int16_t rebin_short(const DevGeom &g, const int16_t *f, size_t fplane, int v, int col) {
+    const int DE = g.def_elements;
+    const int16_t *p = f + fplane * (size_t)g.def_angles * DE;
+    int16_t out = 0;
+    for (int xr = 0; xr < g.x_rebin; xr++) {
+        int16_t s = p[(size_t)(v * g.v_rebin) * DE + col * g.x_rebin + xr];
+        for (int vr = 1; vr < g.v_rebin; vr++)
+            s = (int16_t)(s + p[(size_t)(v * g.v_rebin + vr) * DE + col * g.x_rebin + xr]);
+        out = xr == 0 ? s : (int16_t)(out + s);
+    }
+    return out;
+}
+
+// mask of file (view, bin) with a positive norm in some non-killed plane of segment th (FIO:158-160)
+__global__ void gapmask_kernel(const DevGeom g, const float *__restrict__ norm_f, uint8_t *__restrict__ gm) {
+    const int th = blockIdx.y;
+    const size_t fp = (size_t)g.def_angles * g.def_elements;
+    const int np = g.yt[th] - g.yb[th] + 1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < fp; i += (size_t)gridDim.x * blockDim.x) {
+        uint8_t m = 0;
+        for (int pl = 0; pl < np; pl++) {
+            if (pl < g.npskip || pl + g.npskip >= np) continue;
+            if (norm_f[((size_t)g.file_seg[th] + pl) * fp + i] > 0.0f) m = 1;
+        }
+        gm[th * fp + i] = m;
+    }
+}
+
+__global__ void stage_kernel(const DevGeom g, StageArgs a) {
+    const int N = g.N, V = g.views;
+    const size_t total = (size_t)g.S * V * N;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int col = (int)(idx % N);
+        const int v = (int)((idx / N) % V);
+        const int p = (int)(idx / ((size_t)N * V));
+        const int th = a.plane_seg[p];
+        const int i = p - g.segment_offset[th];
+        const int np = g.yt[th] - g.yb[th] + 1;
+        const size_t fplane = (size_t)g.file_seg[th] + i;
+        const int rebin = g.x_rebin * g.v_rebin;
+        // norm (read with norm_flag = 1, FIO:110-180)
+        float nrm = 1.0f;
+        if (a.norm_f) {
+            if (i < g.npskip || i + g.npskip >= np) {
+                nrm = 0.0f;
+            } else if (rebin == 1) {
+                nrm = rebin_sum(g, a.norm_f, fplane, v, col);
+            } else {
+                nrm = rebin_norm(g, a.norm_f, fplane, v, col);
+                const size_t fp = (size_t)g.def_angles * g.def_elements;
+                int cnt = 0;
+                for (int vr = 0; vr < g.v_rebin; vr++)
+                    for (int xr = 0; xr < g.x_rebin; xr++)
+                        cnt += a.gapmask[th * fp + (size_t)(v * g.v_rebin + vr) * g.def_elements + col * g.x_rebin + xr];
+                if (cnt < rebin) nrm = 0.0f;
+            }
+        }
+        const float d = a.delayed_f ? rebin_sum(g, a.delayed_f, fplane, v, col) : 0.0f;
+        const float at = a.atten_f ? rebin_sum(g, a.atten_f, fplane, v, col) : 1.0f;
+        int16_t pr = 0;
+        if (a.prompt_f) pr = rebin_short(g, a.prompt_f, fplane, v, col);
+        float ad = d * nrm;                                     // mulprojection S3:1715
+        if (nrm == 0.0f) pr = 0;                                // S3:1720-1724
+        if (a.scatter_f) ad = ad + rebin_sum(g, a.scatter_f, fplane, v, col);  // addprojection S3:1753
+        ad = ad * at;                                           // mulprojection S3:1769
+        float inv;                                              // norm_ANW S3:809-817
+        const float am = at < 1.0f ? 1.0f : at;
+        if (nrm > 0.0f) {
+            const float den = nrm * am;
+            inv = den > g.eps2 ? 1.0f / den : 0.0f;
+        } else {
+            inv = nrm;
+        }
+        if (nrm != 0.0f) atomicOr(&a.colmask[v * N + col], 1);
+        const int pair = v % (V / 2), h = v / (V / 2);
+        const size_t o = ((size_t)(pair * 2 + h) * N + col) * g.spp + g.seg_data[th] + i;
+        a.prompt[o] = pr;
+        a.add[o] = ad;
+        a.invnorm[o] = inv;
+    }
+}
+
+// norm_zero_mask S3:899-909
+__global__ void nzmask_kernel(const DevGeom g, const int *__restrict__ colmask, int8_t *__restrict__ nz) {
+    const int v = blockIdx.x, N = g.N, V = g.views;
+    for (int xr = threadIdx.x; xr <= N / 2; xr += blockDim.x) {
+        int8_t m = 1;
+        if (xr >= 1) {
+            const bool z = colmask[v * N + xr] == 0 && colmask[(v + V / 2) * N + xr] == 0 &&
+                           colmask[v * N + N - xr] == 0 && colmask[(v + V / 2) * N + N - xr] == 0;
+            m = z ? 0 : 1;
+        }
+        nz[v * (N / 2 + 1) + xr] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+template <int MAXT>
+static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float *image, float *out,
+                              const int16_t *prompt, const float *add, int ratio, hipStream_t s) {
+    int threads = (g.ntasks + MAXT - 1) / MAXT;
+    threads = ((threads + 63) / 64) * 64;
+    if (threads > 512) return hipErrorInvalidConfiguration;
+    dim3 grid(g.c, b.npairs), block(threads);
+    size_t shm = sizeof(float4) * 5 * (g.Z4 + 4);
+    if (ratio)
+        hipLaunchKernelGGL((fp_kernel<MAXT, true>), grid, block, shm, s, g, b,
+                           reinterpret_cast<const float4 *>(image), out, prompt, add);
+    else
+        hipLaunchKernelGGL((fp_kernel<MAXT, false>), grid, block, shm, s, g, b,
+                           reinterpret_cast<const float4 *>(image), out, prompt, add);
+    return hipGetLastError();
+}
+
+hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *image, float *out,
+                          const int16_t *prompt, const float *add, int ratio_mode, hipStream_t s) {
+    // choose the smallest task count per thread that keeps the workgroup within 512 threads
+    if ((g.ntasks + 0) <= 512 * 1 && g.ntasks <= 256) return fp_launch_t<1>(g, b, image, out, prompt, add, ratio_mode, s);
+    if (g.ntasks <= 2 * 256) return fp_launch_t<2>(g, b, image, out, prompt, add, ratio_mode, s);
+    if (g.ntasks <= 3 * 320) return fp_launch_t<3>(g, b, image, out, prompt, add, ratio_mode, s);
+    if (g.ntasks <= 4 * 512) return fp_launch_t<4>(g, b, image, out, prompt, add, ratio_mode, s);
+    return hipErrorInvalidConfiguration;
+}
+
+hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const float *src, int src_global,
+                              int apply_rescale, float *bufA, float *bufB, hipStream_t s) {
+    dim3 grid(g.N - 1, 2, b.npairs), block(256);
+    size_t shm = sizeof(float) * g.spp;
+    hipLaunchKernelGGL(bp_kernel, grid, block, shm, s, g, b, src, src_global, apply_rescale, g.pseg, bufA, bufB);
+    return hipGetLastError();
+}
+
+static dim3 rot_grid(const DevGeom &g) { return dim3(g.N, (g.N + 3) / 4); }
+
+hipError_t launch_rotate_update(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
+                                float *image, const float *normfac, hipStream_t s) {
+    hipLaunchKernelGGL(rot_update_kernel, rot_grid(g), dim3(64, 4), 0, s, g, b,
+                       reinterpret_cast<const float4 *>(bufA), reinterpret_cast<const float4 *>(bufB),
+                       reinterpret_cast<float4 *>(image), reinterpret_cast<const float4 *>(normfac));
+    return hipGetLastError();
+}
+
+hipError_t launch_rotate_sens(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
+                              float *normfac_out, hipStream_t s) {
+    size_t n4 = (size_t)g.N * g.N * g.NQ;
+    hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, reinterpret_cast<float4 *>(normfac_out), n4);
+    hipLaunchKernelGGL(rot_sens_kernel, rot_grid(g), dim3(64, 4), 0, s, g, b,
+                       reinterpret_cast<const float4 *>(bufA), reinterpret_cast<const float4 *>(bufB),
+                       reinterpret_cast<float4 *>(normfac_out));
+    return hipGetLastError();
+}
+
+hipError_t launch_rotate_add(const DevGeom &g, int view, const float *bufA, const float *bufB, float *ima,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(rot_add_kernel, rot_grid(g), dim3(64, 4), 0, s, g, view,
+                       reinterpret_cast<const float4 *>(bufA), reinterpret_cast<const float4 *>(bufB),
+                       reinterpret_cast<float4 *>(ima));
+    return hipGetLastError();
+}
+
+hipError_t launch_paircol_to_estimate(const DevGeom &g, int view, const float *paircol, float *prj, hipStream_t s) {
+    hipLaunchKernelGGL(paircol_to_estimate_kernel, dim3(g.N, 2), dim3(256), 0, s, g, view, paircol, prj);
+    return hipGetLastError();
+}
+
+hipError_t launch_estimate_to_paircol(const DevGeom &g, float *prj, float *paircol, hipStream_t s) {
+    hipLaunchKernelGGL(estimate_to_paircol_kernel, dim3(g.S, 2), dim3(256), 0, s, g, prj, paircol);
+    return hipGetLastError();
+}
+
+hipError_t launch_ratio_w3(const DevGeom &g, float *est, const float *add, const int16_t *prompt, hipStream_t s) {
+    size_t n = (size_t)2 * g.S * g.N;
+    hipLaunchKernelGGL(ratio_w3_kernel, dim3(1024), dim3(256), 0, s, g, est, add, prompt, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_image(const DevGeom &g, float *image, float *corr, const float *nf, hipStream_t s) {
+    hipLaunchKernelGGL(update_image_kernel, dim3(g.N), dim3(64, 4), 0, s, g, image, corr, nf);
+    return hipGetLastError();
+}
+
+hipError_t launch_normfac_invert(const DevGeom &g, const float *sens, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(normfac_invert_kernel, dim3(g.N), dim3(64, 4), 0, s, g, sens, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_image(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s) {
+    hipLaunchKernelGGL(init_image_kernel, dim3(g.N), dim3(64, 4), 0, s, g, image, z_min, z_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_end_iteration(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s) {
+    hipLaunchKernelGGL(end_iteration_kernel, dim3(g.N), dim3(64, 4), 0, s, g, image, z_min, z_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *delayed_f, const float *norm_f,
+                        const float *scatter_f, const float *atten_f, int16_t *prompt, float *add, float *invnorm,
+                        uint8_t *gapmask, int *colmask, hipStream_t s) {
+    const int rebin = g.x_rebin * g.v_rebin;
+    if (norm_f && rebin > 1) {
+        hipLaunchKernelGGL(gapmask_kernel, dim3(256, g.th), dim3(256), 0, s, g, norm_f, gapmask);
+    }
+    StageArgs a{prompt_f, delayed_f, norm_f, scatter_f, atten_f, prompt, add, invnorm, gapmask, colmask, g.plane_seg};
+    hipLaunchKernelGGL(stage_kernel, dim3(8192), dim3(256), 0, s, g, a);
+    hipLaunchKernelGGL(nzmask_kernel, dim3(g.views / 2), dim3(256), 0, s, g, colmask, g.nzmask);
+    return hipGetLastError();
+}
+
+}  // namespace hrrt

@@ -1,0 +1,170 @@
+"""Host-side driver of the OSEM hot path on device memory.
+
+Mirrors the reference driver of hrrt_osem3d (hrrt_osem3d/hrrt_osem3d_span3.cpp):
+GetScannerParameter/dependencies -> ``Osem3D(...)``, PrepareOsem3dFiles +
+PrepareNormfac -> ``stage``, CalculateNormfac -> ``calculate_normfac``,
+Init_Image -> ``init_image``, CalculateOsem3d -> ``calculate_osem3d``, and the
+projector interface (hrrt_osem3d/hrrt_osem3d.h:197-211) -> ``forward_proj3d`` /
+``back_proj3d``.  Device memory is held in torch tensors; every computation
+runs in libhrrt_osem.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+HRRT_MODEL = dict(rings=104, crystal_radius=234.5, plane_sep=1.21875, bin_size=1.21875,
+                  def_elements=256, def_angles=288)          # scanner_model.cpp:100
+RECON_DEFAULTS = dict(subsets=16, rel_fov=0.95, npskip=3, z_min=6, z_max=200,
+                      epsilon1=5e-2, epsilon2=1e-2, epsilon3=1e-10, image_max=1e4)  # S3:231-374
+
+
+def make_params(**kw) -> _lib.Params:
+    d = dict(HRRT_MODEL)
+    d.update(RECON_DEFAULTS)
+    d.update(dict(x_pixels=0, span=9, maxdel=67))
+    d.update(kw)
+    return _lib.Params(**d)
+
+
+def _p(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Osem3D:
+    """One reconstruction context (geometry tables resident on the device)."""
+
+    def __init__(self, device: str = "cuda:0", **params):
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        self.params = make_params(**params)
+        h = C.c_void_p()
+        _lib.call("hrrt_create", C.byref(self.params), C.byref(h))
+        self.ctx = h
+        d = _lib.Dims()
+        _lib.call("hrrt_get_dims", self.ctx, C.byref(d))
+        self.dims = d
+        self.N, self.views, self.Z, self.Z4 = d.N, d.views, d.z_pixels, d.z4
+        self.S, self.spp, self.subsets, self.sviews = d.segmentsize, d.spp, d.subsets, d.sviews
+        order = np.zeros(self.subsets, np.int32)
+        vo = np.zeros(self.subsets * self.sviews, np.int32)
+        _lib.call("hrrt_get_order", self.ctx, order.ctypes.data_as(C.c_void_p), vo.ctypes.data_as(C.c_void_p))
+        self.order = order
+        self.vieworder = vo.reshape(self.subsets, self.sviews)
+        th = d.th_pixels
+        seg = [np.zeros(th, np.int32) for _ in range(4)]
+        _lib.call("hrrt_get_segments", self.ctx, *[a.ctypes.data_as(C.c_void_p) for a in seg])
+        self.yr_bottom, self.yr_top, self.segment_offset, self.seg_data = seg
+
+    def paircol_index(self):
+        """(plane index in [S], padded index in a pair-column) for every internal plane."""
+        planes, pidx = [], []
+        for s in range(len(self.seg_data)):
+            n = int(self.yr_top[s] - self.yr_bottom[s] + 1)
+            planes.append(np.arange(n) + self.segment_offset[s])
+            pidx.append(np.arange(n) + self.seg_data[s])
+        return np.concatenate(planes), np.concatenate(pidx)
+
+    def sino_to_paircol(self, sino):
+        """[S][views][N] (internal plane order) -> [views/2][2][N][spp] (host numpy)."""
+        planes, pidx = self.paircol_index()
+        V, N = self.views, self.N
+        out = np.zeros((V // 2, 2, N, self.spp), sino.dtype)
+        s2 = sino.reshape(self.S, 2, V // 2, N)          # view = h*V/2 + pair
+        out[:, :, :, pidx] = np.transpose(s2[planes], (2, 1, 3, 0))
+        return out
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            _lib.lib().hrrt_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- buffers ----
+    def image_zeros(self):
+        return torch.zeros((self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)
+
+    def estimate_zeros(self):
+        return torch.zeros((2 * self.S, self.N), dtype=torch.float32, device=self.device)
+
+    def paircol_zeros(self, dtype=torch.float32):
+        return torch.zeros((self.views // 2, 2, self.N, self.spp), dtype=dtype, device=self.device)
+
+    # ---- staging (PrepareOsem3dFiles S3:3876 + PrepareNormfac S3:3726) ----
+    def stage(self, prompt_file, delayed_file, norm_file, scatter_file, atten_file):
+        prompt = self.paircol_zeros(torch.int16)
+        add = self.paircol_zeros()
+        invnorm = self.paircol_zeros()
+        _lib.call("hrrt_stage_sinograms", self.ctx, _p(prompt_file), _p(delayed_file), _p(norm_file),
+                  _p(scatter_file), _p(atten_file), _p(prompt), _p(add), _p(invnorm), _stream())
+        return prompt, add, invnorm
+
+    def norm_zero_mask(self):
+        m = np.zeros((self.views // 2, self.N // 2 + 1), np.int8)
+        _lib.call("hrrt_get_norm_zero_mask", self.ctx, m.ctypes.data_as(C.c_void_p))
+        return m
+
+    def set_norm_zero_mask(self, m):
+        m = np.ascontiguousarray(m, dtype=np.int8)
+        _lib.call("hrrt_set_norm_zero_mask", self.ctx, m.ctypes.data_as(C.c_void_p))
+
+    # ---- sensitivity (CalculateNormfac S3:3760) ----
+    def calculate_normfac(self, invnorm):
+        nf = torch.zeros((self.subsets, self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)
+        _lib.call("hrrt_calculate_normfac", self.ctx, _p(invnorm), _p(nf), _stream())
+        return nf
+
+    # ---- OSEM (CalculateOsem3d S3:4098) ----
+    def init_image(self):
+        img = torch.empty((self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)
+        _lib.call("hrrt_init_image", self.ctx, _p(img), _stream())
+        return img
+
+    def osem_subset(self, image, normfac, prompt, add, isubset: int):
+        _lib.call("hrrt_osem_subset", self.ctx, _p(image), _p(normfac[isubset]), _p(prompt), _p(add),
+                  int(isubset), _stream())
+
+    def end_iteration(self, image):
+        _lib.call("hrrt_end_iteration", self.ctx, _p(image), _stream())
+
+    def calculate_osem3d(self, image, normfac, prompt, add, iterations: int = 1):
+        for _ in range(iterations):
+            for i in range(self.subsets):
+                self.osem_subset(image, normfac, prompt, add, int(self.order[i]))
+            self.end_iteration(image)
+        return image
+
+    # ---- projector interface (H:197-211) ----
+    def forward_proj3d(self, image, view: int, prj=None):
+        prj = self.estimate_zeros() if prj is None else prj
+        _lib.call("hrrt_forward_proj3d", self.ctx, _p(image), _p(prj), int(view), _stream())
+        return prj
+
+    def back_proj3d(self, prj, ima, view: int):
+        _lib.call("hrrt_back_proj3d", self.ctx, _p(prj), _p(ima), int(view), _stream())
+        return ima
+
+    def update_estimate_w3(self, est, add, prompt):
+        _lib.call("hrrt_update_estimate_w3", self.ctx, _p(est), _p(add), _p(prompt), _stream())
+        return est
+
+    def cal_updateimage(self, image, correction, normfac):
+        _lib.call("hrrt_cal_updateimage", self.ctx, _p(image), _p(correction), _p(normfac), _stream())
+
+    def normfac_calculate(self, sens):
+        out = self.image_zeros()
+        _lib.call("hrrt_normfac_calculate", self.ctx, _p(sens), _p(out), _stream())
+        return out

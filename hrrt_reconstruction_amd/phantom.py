@@ -1,0 +1,140 @@
+"""Seeded synthetic HRRT studies in the reference's flat file layouts.
+
+Image phantom: a uniform cylinder (radius 100 mm, activity 1) over the initial
+image support z in [z_min, z_max), holding six hot spheres (radius 5-15 mm,
+activity 4).  Sinograms are produced in the flat file layout
+[file plane][def_angles][def_elements] that hrrt_osem3d reads
+(hrrt_osem3d/file_io_processor.cpp:110-259):
+
+* norm   N = exp(0.1*eps), eps ~ N(0,1) per bin (log-normal around 1)
+* atten  A = exp(0.0096/mm * L) with L the chord of a water cylinder of
+           radius 100 mm along the LOR (2*sqrt(100^2 - s^2)/cos(theta))
+* trues  t = alpha * (P x) / (N*A), P the forward projector in file geometry
+* randoms d uniform, 10 % of the expected counts
+* scatter (optional) s = N * blur(t) scaled to 15 % of the expected counts,
+           blur = Gaussian along the radial axis (sigma 25 mm)
+* prompts y ~ Poisson(t + d + s/N) as int16 (asserted < 32768)
+
+All randomness comes from one torch.Generator seeded by the caller.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+WATER_MU = 0.0096   # 1/mm
+CYL_R = 100.0       # mm
+
+
+def file_segments(rings: int, span: int, maxdel: int):
+    """Plane counts and group of each file segment, in file order (PJ:159-191, 446-457)."""
+    groupmax = (maxdel - (span - 1) // 2) // span
+    segs = [(2 * rings - 1, 0)]
+    for g in range(1, groupmax + 1):
+        zmin = (span + 1) // 2 + (g - 1) * span
+        n = 2 * rings - 1 - 2 * zmin
+        segs += [(n, g), (n, g)]
+    return segs
+
+
+def make_phantom(N, Z4, z_pixels, pixel_mm, z_mm, z_min, z_max, gen, device, activity_scale=1.0,
+                 sphere_activity=None):
+    """Cylinder + 6 spheres in the [X][Y][Z4] image layout.  Returns (image, spheres)."""
+    c = N // 2
+    r = torch.rand(6, generator=gen, device="cpu") * 10.0 + 5.0
+    spheres = []
+    zc_lo, zc_hi = (z_min + 2) * z_mm, (z_max - 3) * z_mm
+    for i in range(6):
+        while True:
+            u = torch.rand(3, generator=gen, device="cpu")
+            rad = float(r[i])
+            rho = (CYL_R - rad - 3.0) * math.sqrt(float(u[0]))
+            ang = 2 * math.pi * float(u[1])
+            zc = zc_lo + rad + (zc_hi - zc_lo - 2 * rad) * float(u[2])
+            spheres.append((rho * math.cos(ang), rho * math.sin(ang), zc, rad))
+            break
+    xs = (torch.arange(N, device=device, dtype=torch.float64) - c) * pixel_mm
+    zs = torch.arange(Z4, device=device, dtype=torch.float64) * z_mm
+    X = xs.view(N, 1, 1)
+    Y = xs.view(1, N, 1)
+    Zc = zs.view(1, 1, Z4)
+    zi = torch.arange(Z4, device=device).view(1, 1, Z4)
+    inz = (zi >= z_min) & (zi < z_max)
+    img = ((X * X + Y * Y <= CYL_R * CYL_R) & inz).to(torch.float64)
+    for k, (sx, sy, sz, sr) in enumerate(spheres):
+        a = 4.0 if sphere_activity is None else float(sphere_activity[k])
+        inside = ((X - sx) ** 2 + (Y - sy) ** 2 + (Zc - sz) ** 2 <= sr * sr) & inz
+        img = torch.where(inside, torch.full_like(img, a), img)
+    return (img * activity_scale).to(torch.float32).contiguous(), spheres
+
+
+def atten_file(def_elements, def_angles, bin_mm, segs, ring_radius, ring_spacing, span, device):
+    """exp(mu*L) in file layout (float32)."""
+    s = (torch.arange(def_elements, device=device, dtype=torch.float64) - def_elements // 2) * bin_mm
+    chord = 2.0 * torch.sqrt(torch.clamp(CYL_R * CYL_R - s * s, min=0.0))
+    planes = []
+    for n, g in segs:
+        t = g * span * ring_spacing
+        cos_t = (2 * ring_radius) / math.sqrt(t * t + (2 * ring_radius) ** 2)
+        a = torch.exp(WATER_MU * chord / cos_t).to(torch.float32)
+        planes.append(a.view(1, 1, def_elements).expand(n, def_angles, def_elements))
+    return torch.cat(planes, 0).contiguous()
+
+
+def gaussian_blur_radial(x, sigma_bins):
+    """Blur along the last axis with a normalised Gaussian (zero padding)."""
+    half = int(math.ceil(3 * sigma_bins))
+    k = torch.exp(-0.5 * (torch.arange(-half, half + 1, device=x.device, dtype=torch.float32) / sigma_bins) ** 2)
+    k = (k / k.sum()).view(1, 1, -1)
+    flat = x.reshape(-1, 1, x.shape[-1])
+    out = torch.empty_like(flat)
+    step = 1 << 20
+    for i in range(0, flat.shape[0], step):
+        out[i:i + step] = torch.nn.functional.conv1d(flat[i:i + step], k, padding=half)
+    return out.view_as(x)
+
+
+def make_study(proj_file, rings, span, maxdel, def_elements, def_angles, bin_mm, plane_sep, ring_radius,
+               total_counts, gen, device, with_scatter=False, norm_sigma=0.1, randoms_frac=0.10,
+               scatter_frac=0.15):
+    """Build file-layout sinograms for an already forward-projected phantom.
+
+    proj_file: (P x) in file layout [S][def_angles][def_elements] (float32, device).
+    Returns dict(prompt=int16, delayed=f32, norm=f32, atten=f32, scatter=f32 or None, alpha=float).
+    """
+    segs = file_segments(rings, span, maxdel)
+    ring_spacing = 2.0 * plane_sep
+    shape = proj_file.shape
+    norm = torch.exp(norm_sigma * torch.randn(shape, generator=gen, device=device, dtype=torch.float32))
+    atten = atten_file(def_elements, def_angles, bin_mm, segs, ring_radius, ring_spacing, span, device)
+    base = proj_file / (norm * atten)                 # trues per unit activity
+    sfrac = scatter_frac if with_scatter else 0.0
+    tfrac = 1.0 - randoms_frac - sfrac
+    alpha = tfrac * total_counts / float(base.sum(dtype=torch.float64))
+    trues = base * alpha
+    nbins = proj_file.numel()
+    d = torch.full(shape, randoms_frac * total_counts / nbins, device=device, dtype=torch.float32)
+    mean = trues + d
+    scatter = None
+    if with_scatter:
+        b = gaussian_blur_radial(trues, 25.0 / bin_mm)
+        sc = b * (sfrac * total_counts / float(b.sum(dtype=torch.float64)))
+        mean = mean + sc
+        scatter = (sc * norm).contiguous()
+        del b, sc
+    prompt = torch.poisson(mean, generator=gen)
+    mx = float(prompt.max())
+    assert mx < 32768, f"prompt counts {mx} overflow the reference's int16 sinograms"
+    prompt = prompt.to(torch.int16).contiguous()
+    return dict(prompt=prompt, delayed=d.contiguous(), norm=norm.contiguous(), atten=atten, scatter=scatter,
+                alpha=alpha)
+
+
+def estimates_to_file(estimates_by_view, S, views, N, device):
+    """Assemble per-view-pair estimates [2S][N] into the file layout [S][views][N]."""
+    out = torch.zeros((S, views, N), dtype=torch.float32, device=device)
+    for v, e in estimates_by_view.items():
+        out[:, v, :] = e[:S]
+        out[:, v + views // 2, :] = e[S:]
+    return out

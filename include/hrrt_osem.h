@@ -1,0 +1,164 @@
+/*
+ * hrrt_osem.h -- C ABI of the MI355X (gfx950) OSEM hot path of hrrt_osem3d.
+ *
+ * The reference's projector interface is a set of C++ functions over
+ * pointer-to-pointer arrays plus process globals (hrrt_osem3d/hrrt_osem3d.h:197-211).
+ * This ABI keeps its operations, argument meaning and layouts, replaces the
+ * globals by an explicit context, and reports errors through status codes
+ * instead of exit(1) (the reference's LOG_EXIT, hrrt_common/my_spdlog.hpp:58-61).
+ *
+ * Citations: PJ = hrrt_osem3d/hrrt_osem3dv_sbrt_span3.cpp,
+ *            S3 = hrrt_osem3d/hrrt_osem3d_span3.cpp,
+ *            FIO = hrrt_osem3d/file_io_processor.cpp.
+ *
+ * Layouts (all contiguous, little endian):
+ *   image        float [X][Y][Z4], Z4 = 4*ceil(z_pixels/4)           (matrix3dm128, nr_utils.cpp:248)
+ *   file sino    [file plane][def_angles][def_elements]              (FIO:110-259; S3:3706-3719)
+ *   estimate     float [2*S][N]: rows 0..S-1 view v, S..2S-1 view v+views/2 (S3:2142, 2145-2154)
+ *   pair-column  device staging layout of a sinogram:
+ *                [views/2 pairs][2 halves][N columns][spp padded planes]
+ *                (half 0 = view v, half 1 = view v+views/2; segment s plane yr at
+ *                 hrrt_dims.seg_data[s] + yr - yr_bottom[s]; zero guard plane on each side)
+ *   normfac      float [subsets][X][Y][Z4]                           (normfac.i, FIO:360-427)
+ *
+ * All "d_" pointers are device pointers on the current HIP device; `stream`
+ * is a hipStream_t (NULL = default stream).  Calls are asynchronous with
+ * respect to the host unless stated otherwise; a context is used by one
+ * stream at a time (calls on one context must not overlap).
+ */
+#ifndef HRRT_OSEM_H
+#define HRRT_OSEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HRRT_OK = 0,
+    HRRT_ERR_ARG = 1,        /* invalid argument / geometry                      */
+    HRRT_ERR_HIP = 2,        /* HIP runtime failure                              */
+    HRRT_ERR_NOMEM = 3,      /* device or host allocation failed                 */
+    HRRT_ERR_UNSUPPORTED = 4 /* option the reference supports but this build does not */
+};
+
+/* Scanner model + reconstruction options (GLOBAL_OSEM3DPAR S3:387-445,
+ * ScannerModel scanner_model.h:3-16). */
+typedef struct {
+    int   rings;           /* crystal rings (104)                        */
+    float crystal_radius;  /* mm (234.5)                                 */
+    float plane_sep;       /* mm (1.21875)                               */
+    float bin_size;        /* mm (1.21875)                               */
+    int   def_elements;    /* radial bins in the files (256)             */
+    int   def_angles;      /* views in the files (288)                   */
+    int   x_pixels;        /* image size -X (0 = def_elements)           */
+    int   span;            /* -m span                                    */
+    int   maxdel;          /* -m ,maxdel                                 */
+    int   subsets;         /* -S                                         */
+    float rel_fov;         /* -f rel_fov (0.95)                          */
+    int   npskip;          /* planes killed at segment ends (3)          */
+    int   z_min, z_max;    /* -f ,,z_min,z_max (6, 200)                  */
+    float epsilon1, epsilon2, epsilon3, image_max; /* -e, -u              */
+} hrrt_params;
+
+typedef struct {
+    int N;             /* radial bins = image X = Y                       */
+    int views;         /* views (pairs = views/2)                         */
+    int z_pixels, z4;  /* 207, 208                                        */
+    int segmentsize;   /* S: planes of all segments (2209 span 9, 6367 span 3) */
+    int th_pixels;     /* segments                                        */
+    int sviews;        /* views per subset                                */
+    int subsets;
+    int spp;           /* padded planes per pair-column                   */
+    int x_rebin, v_rebin;
+    long long image_floats;       /* X*Y*Z4                               */
+    long long sino_bins;          /* S*views*N (internal sinogram)        */
+    long long file_bins;          /* S*def_angles*def_elements (file)     */
+    long long paircol_elems;      /* (views/2)*2*N*spp                    */
+} hrrt_dims;
+
+/* Create a context: GetScannerParameter (S3:3452) + dependencies (PJ:199).
+ * Builds every geometry table on the host and uploads it to the current
+ * device.  Returns HRRT_ERR_ARG for geometries the reference does not
+ * support (views % subsets != 0, zoom != 1, groupmax < 1) or for which the
+ * rotated samples could leave the image (FOV radius + 1 >= N/2 - 1). */
+typedef struct hrrt_ctx hrrt_ctx;
+int  hrrt_create(const hrrt_params *params, hrrt_ctx **out);
+void hrrt_destroy(hrrt_ctx *ctx);
+int  hrrt_get_dims(const hrrt_ctx *ctx, hrrt_dims *out);
+/* Segment tables ([th_pixels] each): plane range (yr_bottom/yr_top, PJ:446-457),
+ * plane offset in the [S] internal order (segment_offset) and start of the
+ * segment in a pair-column (seg_data). */
+int  hrrt_get_segments(const hrrt_ctx *ctx, int *yr_bottom, int *yr_top, int *segment_offset, int *seg_data);
+/* Subset order (sequence S3:535) and view order (calculatevieworder PJ:144). */
+int  hrrt_get_order(const hrrt_ctx *ctx, int *order /* [subsets] */, int *vieworder /* [subsets][sviews] */);
+/* norm_zero_mask (S3:864-909): [views/2][N/2+1] int8, 0 = the (v, v+views/2, x, N-x)
+ * columns of the norm are zero in every plane.  Staging computes it; a driver
+ * that prepares the norm itself sets it here. */
+int  hrrt_set_norm_zero_mask(hrrt_ctx *ctx, const int8_t *mask);
+int  hrrt_get_norm_zero_mask(hrrt_ctx *ctx, int8_t *mask);
+/* Thread-local message for the last non-zero status. */
+const char *hrrt_last_error(void);
+
+/* ---------------- staging (file_io_processor + prepare_*) ---------------- */
+/* prepare_osem3dw3 (S3:1649-1818, OP weighting, short prompts, Atten3D) and
+ * prepare_norm's 1/(N*A) (S3:827-1003, norm_ANW S3:796): reads whole flat
+ * files that are already in device memory (file layout; NULL = file absent,
+ * default values as FIO:119-123) and writes pair-column buffers:
+ *   d_prompt  int16  prompts, zeroed where the norm is 0
+ *   d_add     float  (d*N + s)*A
+ *   d_invnorm float  N>0 ? (N*max(A,1) > eps2 ? 1/(N*max(A,1)) : 0) : N
+ * Rebinning (-X 128: x_rebin = v_rebin = 2), norm harmonic rebin with gap
+ * mask and npskip end-plane killing follow FIO:29-180.  Also computes the
+ * norm zero-column mask (S3:864-909) kept in the context. */
+int hrrt_stage_sinograms(hrrt_ctx *ctx, const int16_t *d_prompt_file, const float *d_delayed_file,
+                         const float *d_norm_file, const float *d_scatter_file, const float *d_atten_file,
+                         int16_t *d_prompt, float *d_add, float *d_invnorm, void *stream);
+
+/* ---------------- sensitivity (CalculateNormfac S3:3760-3865) ---------------- */
+/* For every subset: back-project the subset's view pairs of d_invnorm (BP
+ * rescale applied on the fly, d_invnorm itself is not modified), rotate and
+ * accumulate, then d_normfac[subset] = |sens| > eps1 ? 1/sens : 0
+ * (pt_normfac_calculate S3:1852).  d_normfac: [subsets][X][Y][Z4]. */
+int hrrt_calculate_normfac(hrrt_ctx *ctx, const float *d_invnorm, float *d_normfac, void *stream);
+
+/* ---------------- OSEM (CalculateOsem3d S3:4098-4341) ---------------- */
+int hrrt_init_image(hrrt_ctx *ctx, float *d_image, void *stream);              /* Init_Image S3:3997     */
+/* One subset update: forward projection of every view pair of `isubset`
+ * (forward_proj3d_thread1 PJ:1373), ratio (update_estimate_w3 S3:2474),
+ * back projection (back_proj3d2 PJ:1457) and rotation (rotate_image_buf
+ * PJ:1701) into the correction, then the image update (pt_cal_updateimage
+ * S3:2018) with d_normfac_subset.  The correction is never materialised. */
+int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subset,
+                     const int16_t *d_prompt, const float *d_add, int isubset, void *stream);
+int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream);           /* S3:4288-4297           */
+
+/* ---------------- per-view-pair projector calls ---------------- */
+/* forward_proj3d_thread1 / forward_proj3d2 (PJ:1271, 1373): estimate of view
+ * pair `view` in the [2S][N] layout.  Bins the reference does not write
+ * (xr = 0, columns outside the FOV, masked columns) are left untouched. */
+int hrrt_forward_proj3d(hrrt_ctx *ctx, const float *d_image, float *d_prj, int view, void *stream);
+/* back_proj3d2 (PJ:1457): rescales d_prj in place (as the reference does),
+ * back-projects both views of the pair and adds the rotated result into d_ima. */
+int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream);
+/* update_estimate_w3 (S3:2474): est = prompt==0 ? 0 : (est+add > eps3 ? prompt/(est+add) : 1),
+ * over the [2S][N] rows of one view pair. */
+int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float *d_add, const int16_t *d_prompt, void *stream);
+/* pt_cal_updateimage (S3:2018): image = min(image_max, image*clamp(corr,0.01,100)*normfac); corr = 0. */
+int hrrt_cal_updateimage(hrrt_ctx *ctx, float *d_image, float *d_correction, const float *d_normfac, void *stream);
+/* pt_normfac_calculate (S3:1852): out = |sens|>eps1 ? 1/sens : 0 over the FOV, z < z_pixels. */
+int hrrt_normfac_calculate(hrrt_ctx *ctx, const float *d_sens, float *d_out, void *stream);
+
+/* ---------------- host-buffer shims (reference layouts, synchronous) ---------------- */
+/* Same operations on host memory in the reference's own layouts; each call
+ * copies in, runs on the device and copies out.  These are what a driver
+ * written against the reference's functions binds to. */
+int hrrt_forward_proj3d_host(hrrt_ctx *ctx, const float *image, float *prj, int view);
+int hrrt_back_proj3d_host(hrrt_ctx *ctx, float *prj, float *ima, int view);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HRRT_OSEM_H */

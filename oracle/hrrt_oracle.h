@@ -1,0 +1,157 @@
+/*
+ * hrrt_oracle.h -- CPU restatement of the hrrt_osem3d OSEM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker that the HIP path is
+ * compared against (tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg).  It is never linked into, loaded by or called from the
+ * product library in hrrt_reconstruction_amd/.
+ *
+ * Every routine restates the arithmetic of the reference (hrrt_osem3d/ in
+ * andrewcrabb/hrrt_reconstruction) with the same float operation order and
+ * no fused multiply-add (the build uses -ffp-contract=off), so that results
+ * can be compared bit for bit where the reference is deterministic.
+ * File:line citations use:
+ *   PJ  = hrrt_osem3d/hrrt_osem3dv_sbrt_span3.cpp
+ *   S3  = hrrt_osem3d/hrrt_osem3d_span3.cpp
+ *   FIO = hrrt_osem3d/file_io_processor.cpp
+ *
+ * Layouts are the reference's own (all contiguous, row-major):
+ *   image      [X][Y][Z4]            Z4 = 4*ceil(z_pixels/4)         (nr_utils.cpp:248)
+ *   estimate   [2*S][N]              rows 0..S-1 view v, S..2S-1 view v+views/2 (S3:2142)
+ *   sinogram   [S][views][N]         internal plane order == file order (FIO:116, PJ:453)
+ *   prjbuf     [N+1][K+1][P][4]      per-x, per-segment-pair float4 lanes (S3:2754)
+ * where N = radial pixels, S = segmentsize (total planes), K = th_pixels/2,
+ * P = planes per prjbuf row (z_pixels + guard planes).
+ */
+#ifndef HRRT_ORACLE_H
+#define HRRT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int   rings;           /* crystal rings (104 for HRRT)                 scanner_model.cpp:100 */
+    float crystal_radius;  /* mm (234.5)                                                        */
+    float plane_sep;       /* mm (1.21875)                                                      */
+    float bin_size;        /* mm (1.21875)                                                      */
+    int   def_elements;    /* radial bins in the file (256)                                     */
+    int   def_angles;      /* views in the file (288)                                           */
+    int   x_pixels;        /* requested image size (-X); 0 keeps def_elements                   */
+    int   span;            /* 3 or 9                                                            */
+    int   maxdel;          /* maximum ring difference (67)                                      */
+    int   subsets;         /* 16                                                                */
+    float rel_fov;         /* 0.95                                                              */
+    int   npskip;          /* 3                                                                 */
+    int   z_min, z_max;    /* 6, 200                                                            */
+    float epsilon1, epsilon2, epsilon3, image_max; /* 5e-2, 1e-2, 1e-10, 1e4                   */
+} og_params;
+
+typedef struct { float zzz0; int zzi; } og_flookup;                       /* H:52  */
+typedef struct { int zsa, zea, yra; float yr2a; } og_blookup;             /* H:57  */
+typedef struct {                                                          /* PJ:91 */
+    int z_start, yr_start, z_end, z_start_pre, yr_start_pre, z_end_pre;
+    float coef, coef_pre;
+    int num;
+} og_blutacc;
+
+typedef struct {
+    og_params p;
+    /* derived scalars (GetScannerParameter S3:3452, dependencies PJ:199) */
+    int   x_rebin, v_rebin;
+    float sino_sampling, z_size, ring_spacing, x_size, y_size, rfov;
+    int   z_pixels, z4, groupmax, th_pixels, N, views, sviews, segmentsize, zshift;
+    int   z_min, z_max;
+    /* tables */
+    int   *yr_bottom, *yr_top;       /* [th_pixels]             */
+    int   *seg_offset;               /* [th_pixels] file offsets in bins */
+    int   *segment_offset;           /* [th_pixels] plane offsets */
+    float *sin_theta, *cos_theta;    /* [th_pixels]             */
+    float *sin_psi, *cos_psi;        /* [views]                 */
+    short *cylwiny;                  /* [N][2]                  */
+    float *vcos, *vsin;              /* [groupmax][zshift]      */
+    float *tantheta, *rescalearray;  /* [groupmax][z_pixels]    */
+    og_flookup *flookup;             /* [N][groupmax*2*zshift]  */
+    og_blookup *blookup;             /* [N][groupmax]           */
+    og_blutacc *blutacc;             /* [N][groupmax]           */
+    int   *blutacc_flag;             /* [N][groupmax]           */
+    int   *vieworder;                /* [subsets][sviews]       */
+    short *order;                    /* [subsets]               */
+    /* per-(view pair, x) mask: 0 where norm is zero for all four lanes (S3:899-909) */
+    short *norm_zero_mask;           /* [views/2][N/2+1]        */
+    /* prjbuf geometry */
+    int   P;                         /* planes per prjbuf row (z4 + 2 guards)     */
+} og_geom;
+
+/* ---- geometry ---- */
+og_geom *og_create(const og_params *p);           /* NULL on invalid geometry */
+void     og_destroy(og_geom *g);
+int      og_sequence(int sets, short *order);     /* S3:535 */
+
+/* ---- projectors (single view pair, reference layouts) ---- */
+/* forward_proj3d_thread1 PJ:1373 (+ forward_proj3d_view1_thread PJ:1728).
+ * prjbuf: caller scratch of og_prjbuf_floats(g) floats (zeroed once).
+ * prj: [2S][N]; only covered bins are written (xr=0 and columns outside the
+ * FOV keep their previous content, PJ:1322,1397). */
+long     og_prjbuf_floats(const og_geom *g);
+void     og_forward_proj(const og_geom *g, const float *image, float *prj, int view, float *prjbuf);
+/* back_proj3d2 PJ:1457 with back_proj3d_view_n1 PJ:2927 (direct two-pass BP)
+ * followed by rotateimage2 into ima (+=).  prj is rescaled in place. */
+void     og_back_proj_direct(const og_geom *g, float *prj, float *ima, int view, float *imagebuf, float *prjbuf);
+/* back_proj3d_thread1 PJ:1622 with back_proj3d_view_n1thread PJ:4162
+ * (incremental BP) into imagebuf (overwrites the disk voxels).  prj is
+ * rescaled in place; prjbuf is used as scratch. */
+void     og_back_proj_incremental(const og_geom *g, float *prj, float *imagebuf, int view, float *prjbuf);
+/* rotateimage2 PJ:1137: out[x][y][:] += bilinear(imagebuf, R(-psi)(x,y)) for x in [xs,xe) */
+void     og_rotate_add(const og_geom *g, const float *imagebuf, float *out, int view, int xs, int xe);
+/* rotated-frame BP only (both passes of the direct variant), no rotation */
+void     og_back_proj_direct_rotframe(const og_geom *g, float *prj, float *imagebuf, int view, float *prjbuf);
+
+/* ---- per-bin / per-voxel ---- */
+/* update_estimate_w3 S3:2474 (short prompts, OP weighting), over nplanes*N bins of one view pair.
+ * est, add: [2S][N] float; prompt: [2S][N] int16. */
+void     og_ratio_w3(const og_geom *g, float *est, const float *add, const int16_t *prompt);
+/* pt_cal_updateimage S3:2018 over all x: image, correction, normfac [X][Y][Z4] */
+void     og_update_image(const og_geom *g, float *image, float *correction, const float *normfac);
+/* pt_normfac_calculate S3:1852: out = |sens|>eps1 ? 1/sens : 0 (z < z_pixels, FOV rows) */
+void     og_normfac_invert(const og_geom *g, const float *sens, float *out);
+/* Init_Image S3:3997 */
+void     og_init_image(const og_geom *g, float *image);
+/* end-of-iteration threshold S3:4288 */
+void     og_end_iteration(const og_geom *g, float *image);
+
+/* ---- staging (FIO:29-259, S3:796-1056, 1649-1818) ----
+ * Inputs are whole flat files already in memory, in file layout
+ * [file plane][def_angles][def_elements]; outputs are internal layout
+ * [S][views][N].  norm_flag semantics follow FIO:110. */
+void     og_read_flat_float(const og_geom *g, const float *file, float *out, int theta, float def_val, int norm_flag, int have_file);
+void     og_read_flat_short(const og_geom *g, const int16_t *file, int16_t *out, int theta);
+/* prepare_norm (ANW/OP branch, Atten3D): out = 1/(N*max(A,1)) as in norm_ANW S3:796;
+ * also fills g->norm_zero_mask (S3:864-909). */
+void     og_prepare_norm(og_geom *g, const float *norm_file, const float *atten_file, float *out);
+/* prepare_osem3dw3 (Atten3D): add = (d*N + s)*A, prompts zeroed where N==0 */
+void     og_prepare_w3(const og_geom *g, const int16_t *prompt_file, const float *delayed_file,
+                       const float *norm_file, const float *scatter_file, const float *atten_file,
+                       int16_t *prompt, float *add);
+
+/* ---- drivers ---- */
+/* Gather one view pair's rows from a [S][views][N] sinogram into [2S][N] (largeprj[v] S3:2149). */
+void     og_gather_pair_f(const og_geom *g, const float *sino, int view, float *out);
+void     og_gather_pair_s(const og_geom *g, const int16_t *sino, int view, int16_t *out);
+/* CalculateNormfac S3:3760: normfac_all [subsets][X][Y][Z4] receives 1/sens per subset.
+ * invnorm: [S][views][N] (= 1/(N*A)); it is rescaled in place, as the reference does.
+ * nthreads selects the reference's split between the *_thread1 and *2 variants
+ * (S3:3796-3825); nthreads<=0 means "direct variant for every pair". */
+void     og_calculate_normfac(const og_geom *g, float *invnorm, float *normfac_all, int nthreads);
+/* One OSEM subset update (CalculateOsem3d body S3:4165-4279) on subset isubset.
+ * nthreads as above; workers>1 runs the view pairs on that many POSIX threads
+ * (the arithmetic does not depend on workers). */
+void     og_osem_subset(const og_geom *g, float *image, float *correction, const float *normfac_subset,
+                        const float *add, const int16_t *prompt, int isubset, int nthreads, int workers);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

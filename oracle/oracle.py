@@ -1,0 +1,351 @@
+"""ctypes binding of the oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker.  The product package never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("rings", C.c_int), ("crystal_radius", C.c_float), ("plane_sep", C.c_float),
+        ("bin_size", C.c_float), ("def_elements", C.c_int), ("def_angles", C.c_int),
+        ("x_pixels", C.c_int), ("span", C.c_int), ("maxdel", C.c_int), ("subsets", C.c_int),
+        ("rel_fov", C.c_float), ("npskip", C.c_int), ("z_min", C.c_int), ("z_max", C.c_int),
+        ("epsilon1", C.c_float), ("epsilon2", C.c_float), ("epsilon3", C.c_float),
+        ("image_max", C.c_float),
+    ]
+
+
+class FLookup(C.Structure):
+    _fields_ = [("zzz0", C.c_float), ("zzi", C.c_int)]
+
+
+class BLookup(C.Structure):
+    _fields_ = [("zsa", C.c_int), ("zea", C.c_int), ("yra", C.c_int), ("yr2a", C.c_float)]
+
+
+class BlutAcc(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("z_start", "yr_start", "z_end", "z_start_pre", "yr_start_pre", "z_end_pre")] + [
+        ("coef", C.c_float), ("coef_pre", C.c_float), ("num", C.c_int)]
+
+
+_PF = C.POINTER(C.c_float)
+_PI = C.POINTER(C.c_int)
+_PS = C.POINTER(C.c_short)
+
+
+class Geom(C.Structure):
+    _fields_ = [
+        ("p", Params),
+        ("x_rebin", C.c_int), ("v_rebin", C.c_int),
+        ("sino_sampling", C.c_float), ("z_size", C.c_float), ("ring_spacing", C.c_float),
+        ("x_size", C.c_float), ("y_size", C.c_float), ("rfov", C.c_float),
+        ("z_pixels", C.c_int), ("z4", C.c_int), ("groupmax", C.c_int), ("th_pixels", C.c_int),
+        ("N", C.c_int), ("views", C.c_int), ("sviews", C.c_int), ("segmentsize", C.c_int),
+        ("zshift", C.c_int), ("z_min", C.c_int), ("z_max", C.c_int),
+        ("yr_bottom", _PI), ("yr_top", _PI), ("seg_offset", _PI), ("segment_offset", _PI),
+        ("sin_theta", _PF), ("cos_theta", _PF), ("sin_psi", _PF), ("cos_psi", _PF),
+        ("cylwiny", _PS), ("vcos", _PF), ("vsin", _PF), ("tantheta", _PF), ("rescalearray", _PF),
+        ("flookup", C.POINTER(FLookup)), ("blookup", C.POINTER(BLookup)),
+        ("blutacc", C.POINTER(BlutAcc)), ("blutacc_flag", _PI),
+        ("vieworder", _PI), ("order", _PS), ("norm_zero_mask", _PS), ("P", C.c_int),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.og_create.restype = C.POINTER(Geom)
+        L.og_create.argtypes = [C.POINTER(Params)]
+        L.og_destroy.argtypes = [C.POINTER(Geom)]
+        L.og_sequence.argtypes = [C.c_int, _PS]
+        L.og_prjbuf_floats.restype = C.c_long
+        L.og_prjbuf_floats.argtypes = [C.POINTER(Geom)]
+        G = C.POINTER(Geom)
+        L.og_forward_proj.argtypes = [G, vp, vp, C.c_int, vp]
+        L.og_back_proj_direct.argtypes = [G, vp, vp, C.c_int, vp, vp]
+        L.og_back_proj_direct_rotframe.argtypes = [G, vp, vp, C.c_int, vp]
+        L.og_back_proj_incremental.argtypes = [G, vp, vp, C.c_int, vp]
+        L.og_rotate_add.argtypes = [G, vp, vp, C.c_int, C.c_int, C.c_int]
+        L.og_ratio_w3.argtypes = [G, vp, vp, vp]
+        L.og_update_image.argtypes = [G, vp, vp, vp]
+        L.og_normfac_invert.argtypes = [G, vp, vp]
+        L.og_init_image.argtypes = [G, vp]
+        L.og_end_iteration.argtypes = [G, vp]
+        L.og_read_flat_float.argtypes = [G, vp, vp, C.c_int, C.c_float, C.c_int, C.c_int]
+        L.og_read_flat_short.argtypes = [G, vp, vp, C.c_int]
+        L.og_prepare_norm.argtypes = [G, vp, vp, vp]
+        L.og_prepare_w3.argtypes = [G, vp, vp, vp, vp, vp, vp, vp]
+        L.og_gather_pair_f.argtypes = [G, vp, C.c_int, vp]
+        L.og_gather_pair_s.argtypes = [G, vp, C.c_int, vp]
+        L.og_calculate_normfac.argtypes = [G, vp, vp, C.c_int]
+        L.og_osem_subset.argtypes = [G, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+HRRT = dict(rings=104, crystal_radius=234.5, plane_sep=1.21875, bin_size=1.21875,
+            def_elements=256, def_angles=288)
+DEFAULTS = dict(subsets=16, rel_fov=0.95, npskip=3, z_min=6, z_max=200,
+                epsilon1=5e-2, epsilon2=1e-2, epsilon3=1e-10, image_max=1e4)
+
+
+def make_params(**kw) -> Params:
+    d = dict(HRRT)
+    d.update(DEFAULTS)
+    d.update(dict(x_pixels=0, span=9, maxdel=67))
+    d.update(kw)
+    return Params(**d)
+
+
+class Oracle:
+    """Owns one og_geom and exposes the restated reference routines on numpy arrays."""
+
+    def __init__(self, **kw):
+        self.L = lib()
+        self.params = make_params(**kw)
+        self.gp = self.L.og_create(C.byref(self.params))
+        if not self.gp:
+            raise ValueError("invalid geometry")
+        g = self.gp.contents
+        self.g = g
+        self.N, self.Z, self.Z4, self.S = g.N, g.z_pixels, g.z4, g.segmentsize
+        self.views, self.th, self.gm, self.zs = g.views, g.th_pixels, g.groupmax, g.zshift
+        self.sviews = g.sviews
+        self.subsets = self.params.subsets
+
+    def close(self):
+        if self.gp:
+            self.L.og_destroy(self.gp)
+            self.gp = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- tables ----
+    def arr(self, name, n, dtype=None):
+        p = getattr(self.g, name)
+        a = np.ctypeslib.as_array(p, shape=(n,)).copy()
+        return a if dtype is None else a.astype(dtype)
+
+    @property
+    def yr_bottom(self):
+        return self.arr("yr_bottom", self.th)
+
+    @property
+    def yr_top(self):
+        return self.arr("yr_top", self.th)
+
+    @property
+    def segment_offset(self):
+        return self.arr("segment_offset", self.th)
+
+    @property
+    def seg_offset(self):
+        return self.arr("seg_offset", self.th)
+
+    @property
+    def cylwiny(self):
+        return self.arr("cylwiny", 2 * self.N).reshape(self.N, 2)
+
+    @property
+    def sin_psi(self):
+        return self.arr("sin_psi", self.views)
+
+    @property
+    def cos_psi(self):
+        return self.arr("cos_psi", self.views)
+
+    @property
+    def sin_theta(self):
+        return self.arr("sin_theta", self.th)
+
+    @property
+    def cos_theta(self):
+        return self.arr("cos_theta", self.th)
+
+    @property
+    def tantheta(self):
+        return self.arr("tantheta", self.gm * self.Z).reshape(self.gm, self.Z)
+
+    @property
+    def rescalearray(self):
+        return self.arr("rescalearray", self.gm * self.Z).reshape(self.gm, self.Z)
+
+    @property
+    def order(self):
+        return self.arr("order", self.subsets)
+
+    @property
+    def vieworder(self):
+        return self.arr("vieworder", self.subsets * self.sviews).reshape(self.subsets, self.sviews)
+
+    @property
+    def norm_zero_mask(self):
+        return self.arr("norm_zero_mask", (self.views // 2 + 1) * (self.N // 2 + 1)).reshape(
+            self.views // 2 + 1, self.N // 2 + 1)
+
+    def set_norm_zero_mask(self, m):
+        m = np.ascontiguousarray(m, dtype=np.int16)
+        C.memmove(self.g.norm_zero_mask, m.ctypes.data, m.nbytes)
+
+    def flookup(self):
+        n = self.N * self.gm * 2 * self.zs
+        a = np.ctypeslib.as_array(C.cast(self.g.flookup, C.POINTER(C.c_int32)), shape=(2 * n,)).copy()
+        frac = a[0::2].view(np.float32).reshape(self.N, self.gm, 2 * self.zs)
+        zi = a[1::2].reshape(self.N, self.gm, 2 * self.zs)
+        return frac, zi
+
+    def blookup(self):
+        n = self.N * self.gm
+        a = np.ctypeslib.as_array(C.cast(self.g.blookup, C.POINTER(C.c_int32)), shape=(4 * n,)).copy().reshape(n, 4)
+        return (a[:, 0].reshape(self.N, self.gm), a[:, 1].reshape(self.N, self.gm),
+                a[:, 2].reshape(self.N, self.gm), a[:, 3].view(np.float32).reshape(self.N, self.gm))
+
+    # ---- buffers ----
+    def image_zeros(self):
+        return np.zeros((self.N, self.N, self.Z4), np.float32)
+
+    def prj_zeros(self):
+        return np.zeros((2 * self.S, self.N), np.float32)
+
+    def prjbuf(self):
+        return np.zeros(self.L.og_prjbuf_floats(self.gp), np.float32)
+
+    def sino_zeros(self, dtype=np.float32):
+        return np.zeros((self.S, self.views, self.N), dtype)
+
+    # ---- projectors ----
+    def forward(self, image, view, prj=None, prjbuf=None):
+        prj = self.prj_zeros() if prj is None else prj
+        prjbuf = self.prjbuf() if prjbuf is None else prjbuf
+        self.L.og_forward_proj(self.gp, _ptr(image), _ptr(prj), view, _ptr(prjbuf))
+        return prj
+
+    def back_direct(self, prj, view, ima=None):
+        prj = prj.copy()
+        ima = self.image_zeros() if ima is None else ima
+        ib = self.image_zeros()
+        self.L.og_back_proj_direct(self.gp, _ptr(prj), _ptr(ima), view, _ptr(ib), _ptr(self.prjbuf()))
+        return ima
+
+    def back_direct_rotframe(self, prj, view):
+        prj = prj.copy()
+        ib = self.image_zeros()
+        self.L.og_back_proj_direct_rotframe(self.gp, _ptr(prj), _ptr(ib), view, _ptr(self.prjbuf()))
+        return ib
+
+    def back_incremental_rotframe(self, prj, view):
+        prj = prj.copy()
+        ib = self.image_zeros()
+        self.L.og_back_proj_incremental(self.gp, _ptr(prj), _ptr(ib), view, _ptr(self.prjbuf()))
+        return ib
+
+    def rotate_add(self, imagebuf, view, out=None):
+        out = self.image_zeros() if out is None else out
+        self.L.og_rotate_add(self.gp, _ptr(imagebuf), _ptr(out), view, 0, self.N)
+        return out
+
+    # ---- elementwise ----
+    def ratio_w3(self, est, add, prompt):
+        est = est.copy()
+        self.L.og_ratio_w3(self.gp, _ptr(est), _ptr(np.ascontiguousarray(add, np.float32)),
+                           _ptr(np.ascontiguousarray(prompt, np.int16)))
+        return est
+
+    def update_image(self, image, correction, normfac):
+        self.L.og_update_image(self.gp, _ptr(image), _ptr(correction), _ptr(normfac))
+
+    def normfac_invert(self, sens):
+        out = self.image_zeros()
+        self.L.og_normfac_invert(self.gp, _ptr(sens), _ptr(out))
+        return out
+
+    def init_image(self):
+        img = self.image_zeros()
+        self.L.og_init_image(self.gp, _ptr(img))
+        return img
+
+    def end_iteration(self, image):
+        self.L.og_end_iteration(self.gp, _ptr(image))
+
+    # ---- staging ----
+    def read_flat_float(self, file, theta, def_val=0.0, norm_flag=0):
+        sw = theta + 1 if theta % 2 == 1 else (0 if theta == 0 else theta - 1)
+        npl = int(self.yr_top[sw] - self.yr_bottom[sw] + 1)
+        out = np.zeros((npl, self.views, self.N), np.float32)
+        self.L.og_read_flat_float(self.gp, _ptr(file), _ptr(out), theta, def_val, norm_flag, 0 if file is None else 1)
+        return out
+
+    def read_flat_short(self, file, theta):
+        sw = theta + 1 if theta % 2 == 1 else (0 if theta == 0 else theta - 1)
+        npl = int(self.yr_top[sw] - self.yr_bottom[sw] + 1)
+        out = np.zeros((npl, self.views, self.N), np.int16)
+        self.L.og_read_flat_short(self.gp, _ptr(file), _ptr(out), theta)
+        return out
+
+    def prepare_norm(self, norm_file, atten_file):
+        out = self.sino_zeros()
+        self.L.og_prepare_norm(self.gp, _ptr(norm_file), _ptr(atten_file), _ptr(out))
+        return out
+
+    def prepare_w3(self, prompt_file, delayed_file, norm_file, scatter_file, atten_file):
+        prompt = self.sino_zeros(np.int16)
+        add = self.sino_zeros()
+        self.L.og_prepare_w3(self.gp, _ptr(prompt_file), _ptr(delayed_file), _ptr(norm_file),
+                             _ptr(scatter_file), _ptr(atten_file), _ptr(prompt), _ptr(add))
+        return prompt, add
+
+    def gather_pair(self, sino, view):
+        if sino.dtype == np.int16:
+            out = np.zeros((2 * self.S, self.N), np.int16)
+            self.L.og_gather_pair_s(self.gp, _ptr(sino), view, _ptr(out))
+        else:
+            out = self.prj_zeros()
+            self.L.og_gather_pair_f(self.gp, _ptr(sino), view, _ptr(out))
+        return out
+
+    # ---- drivers ----
+    def calculate_normfac(self, invnorm, nthreads=0):
+        nf = np.zeros((self.subsets, self.N, self.N, self.Z4), np.float32)
+        self.L.og_calculate_normfac(self.gp, _ptr(invnorm), _ptr(nf), nthreads)
+        return nf
+
+    def osem_subset(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
+        self.L.og_osem_subset(self.gp, _ptr(image), _ptr(correction), _ptr(normfac_subset), _ptr(add),
+                              _ptr(prompt), isubset, nthreads, workers)
