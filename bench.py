@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""OSEM subset-iterations/s on one MI355X: HRRT span 3, 256x256x207, 16 subsets.
+
+Workload (synthetic, seeded): the HRRT geometry (104 rings, 256 bins x 288
+views, span 3, maximum ring difference 67 -> 6367 planes), a 256x256x207
+image at 1.21875 mm holding a cylinder (r = 100 mm) with six 4:1 hot spheres,
+prompts Poisson-sampled to 200M counts from the forward projection, log-normal
+norm, water-cylinder attenuation, 10 % uniform randoms and 15 % Gaussian
+scatter.  The files are staged on the device and the sensitivity images are
+built before the timed region.
+
+A step is one OSEM subset update (forward projection of the subset's 9 view
+pairs, OP ratio, back projection, rotation and image update), the reference's
+CalculateOsem3d inner body.  Steps cycle through the reference's subset order.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress of the setup phases on stderr (the JSON result line goes to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+GEOM = dict(x_pixels=256, span=3, maxdel=67)
+
+
+def algorithmic_bytes(g, npairs):
+    """Bytes each kernel must move per launch (one subset), independent of the implementation."""
+    bins = npairs * 2 * g.S * g.N           # sinogram bins of the subset
+    vol = g.N * g.N * g.Z4 * 4               # one image volume in bytes
+    return {
+        # image read once, int16 prompt + float additive term read, float ratio written per bin
+        "fp_ratio": vol + bins * (2 + 4 + 4),
+        # ratio read per bin, one rotated-frame volume written per view pair
+        "backproject": bins * 4 + npairs * vol,
+        # rotated-frame volumes read, image + 1/sensitivity read, image written
+        "rotate_update": npairs * vol + 3 * vol,
+    }
+
+
+def read_traffic(kernel):
+    """Measured HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic_c2.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == kernel:
+                try:
+                    return float(parts[1]) + float(parts[2])
+                except ValueError:
+                    return None
+    return None
+
+
+def build_study(g, torch, seed):
+    from hrrt_reconstruction_amd import phantom as ph
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    cpu_gen = torch.Generator()
+    cpu_gen.manual_seed(seed)
+    p = g.params
+    img, spheres = ph.make_phantom(g.N, g.Z4, g.Z, p.bin_size, p.plane_sep, 6, 200, cpu_gen, "cuda")
+    est = g.estimate_zeros()
+    proj = torch.zeros((g.S, g.views, g.N), dtype=torch.float32, device="cuda")
+    for v in range(g.views // 2):
+        g.forward_proj3d(img, v, prj=est)
+        proj[:, v, :] = est[: g.S]
+        proj[:, v + g.views // 2, :] = est[g.S:]
+        if v % 48 == 47:
+            torch.cuda.synchronize()
+            log(f"phantom forward projection: {v + 1}/{g.views // 2} view pairs")
+    log("sinogram factors ...")
+    study = ph.make_study(proj, p.rings, p.span, p.maxdel, p.def_elements, p.def_angles, p.bin_size,
+                          p.plane_sep, p.crystal_radius, 200e6, gen, "cuda", with_scatter=True)
+    del proj, est
+    torch.cuda.synchronize()
+    log("study sinograms generated")
+    return img, study
+
+
+def paircol_to_sino(g, torch, pc):
+    planes, pidx = g.paircol_index()
+    V = g.views
+    sel = pc[:, :, :, torch.as_tensor(pidx, device=pc.device)]          # [V/2][2][N][S]
+    return sel.permute(3, 1, 0, 2).reshape(g.S, V, g.N).contiguous()
+
+
+def cpu_baseline(g, torch, prompt, add, normfac, image, nsub, threads):
+    """The oracle's restatement of the reference OSEM subset (reference thread schedule), host cores."""
+    import numpy as np
+    from oracle import oracle
+    oracle.build()
+    o = oracle.Oracle(**GEOM)
+    pr = paircol_to_sino(g, torch, prompt).cpu().numpy()
+    ad = paircol_to_sino(g, torch, add).cpu().numpy()
+    log("cpu baseline: inputs on the host")
+    img = image.cpu().numpy().copy()
+    corr = o.image_zeros()
+    order = [int(s) for s in g.order]
+    # warm-up subset, then the timed ones
+    o.osem_subset(img, corr, np.ascontiguousarray(normfac[order[0]].cpu().numpy()), ad, pr, order[0],
+                  nthreads=threads, workers=threads)
+    log("cpu baseline: warm-up subset done")
+    nfs = [np.ascontiguousarray(normfac[order[1 + i]].cpu().numpy()) for i in range(nsub)]
+    t0 = time.perf_counter()
+    for i in range(nsub):
+        o.osem_subset(img, corr, nfs[i], ad, pr, order[1 + i], nthreads=threads, workers=threads)
+        log(f"cpu baseline: subset {i + 1}/{nsub} done")
+    dt = time.perf_counter() - t0
+    return nsub / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-subsets", type=int, default=4, help="timed subsets of the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(frames and view pairs would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hrrt_reconstruction_amd.osem import Osem3D
+
+    torch.cuda.set_device(0)
+    # plain copy of known size, used to scale the PMC byte counters of the profile runs
+    cal = torch.empty(64 << 20, dtype=torch.float32, device="cuda")
+    cal2 = torch.empty_like(cal)
+    cal2.copy_(cal)
+    torch.cuda.synchronize()
+    del cal, cal2
+
+    log("torch ready")
+    g = Osem3D("cuda:0", **GEOM)
+    log("context created")
+    npairs = g.sviews // 2
+    img_true, study = build_study(g, torch, args.seed)
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    prompt, add, invnorm = g.stage(study["prompt"], study["delayed"], study["norm"], study["scatter"], study["atten"])
+    torch.cuda.synchronize()
+    stage_s = time.perf_counter() - t0
+    log(f"staged sinograms in {stage_s:.2f} s")
+    t0 = time.perf_counter()
+    normfac = g.calculate_normfac(invnorm)
+    torch.cuda.synchronize()
+    normfac_s = time.perf_counter() - t0
+    log(f"sensitivity images in {normfac_s:.2f} s")
+    del study, invnorm
+    torch.cuda.empty_cache()
+
+    image = g.init_image()
+    image0 = image.clone()
+    order = [int(s) for s in g.order]
+    for i in range(args.warmup):
+        g.osem_subset(image, normfac, prompt, add, order[i % g.subsets])
+    torch.cuda.synchronize()
+
+    g.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g.osem_subset(image, normfac, prompt, add, order[(args.warmup + i) % g.subsets])
+        if (args.warmup + i) % g.subsets == g.subsets - 1:
+            g.end_iteration(image)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} subset updates in {elapsed:.3f} s")
+    kt = g.timing_read()
+    g.timing_enable(False)
+
+    value = args.steps / elapsed
+    alg = algorithmic_bytes(g, npairs)
+    dom = max(kt, key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0] / max(kt[dom][1], 1)
+    achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
+    traffic = read_traffic(dom)
+
+    cpu = None
+    if args.cpu_subsets > 0:
+        rate, dt = cpu_baseline(g, torch, prompt, add, normfac, image0, args.cpu_subsets, args.cpu_threads)
+        cpu = {"value": round(rate, 4), "unit": "subset-iters/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"{args.cpu_subsets} OSEM subset updates of the same workload after one warm-up "
+                         f"subset, reference thread schedule T={args.cpu_threads} ({dt:.1f} s)"}
+
+    out = {
+        "metric": "OSEM subset-iters/s span3 256²×207; batched frames/s; % HBM roofline",
+        "value": round(value, 3),
+        "unit": "subset-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "HRRT span 3, 256x256x207 image, 6367-plane 256x288 sinogram, 200M prompts, "
+                               "randoms 10%, scatter 15%, 16 subsets (9 view pairs per subset)",
+                   "subsets": g.subsets, "view_pairs_per_subset": npairs,
+                   "stage_s": round(stage_s, 3), "normfac_s": round(normfac_s, 3)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": traffic},
+        "kernels": {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 4), "launches": v[1],
+                        "alg_bytes": alg[k]} for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

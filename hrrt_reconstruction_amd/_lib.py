@@ -58,6 +58,8 @@ SIGNATURES = {
     "hrrt_init_image": (_I, [_VP, _VP, _VP]),
     "hrrt_osem_subset": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_end_iteration": (_I, [_VP, _VP, _VP]),
+    "hrrt_timing_enable": (_I, [_VP, _I]),
+    "hrrt_timing_read": (_I, [_VP, _VP, _VP]),
     "hrrt_forward_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_back_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_update_estimate_w3": (_I, [_VP, _VP, _VP, _VP, _VP]),

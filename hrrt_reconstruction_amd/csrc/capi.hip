@@ -48,6 +48,29 @@ struct hrrt_ctx {
     uint8_t *gapmask = nullptr;      // [th][def_angles][def_elements]
     float *h_img = nullptr, *h_prj = nullptr;  // device staging for the host shims
     int device = 0;
+    // optional per-kernel timing of hrrt_osem_subset (HIP events on the launch stream)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_marks;   // (kernel id, index of start event)
+
+    hipEvent_t next_event() {
+        if (ev_used == ev_pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev_pool.push_back(e);
+        }
+        return ev_pool[ev_used++];
+    }
+    void mark(int id, hipStream_t s) {
+        if (!timing) return;
+        size_t i = ev_used;
+        hipEvent_t e = next_event();
+        if (e) {
+            (void)hipEventRecord(e, s);
+            if (id >= 0) ev_marks.push_back({id, i});
+        }
+    }
 
     template <typename T>
     int upload(const std::vector<T> &v, const T *&dst) {
@@ -86,6 +109,7 @@ struct hrrt_ctx {
         return HRRT_OK;
     }
     ~hrrt_ctx() {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
         void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, (void *)dg.nzmask};
         for (void *p : s)
@@ -334,9 +358,36 @@ extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_no
     int r;
     if ((r = ctx->ensure_subset_scratch())) return r;
     hrrt::PairBatch b{isubset, G.p.subsets, ctx->npairs()};
+    ctx->mark(0, S(stream));
     HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, d_prompt, d_add, 1, S(stream)));
+    ctx->mark(1, S(stream));
     HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    ctx->mark(2, S(stream));
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, S(stream)));
+    ctx->mark(-1, S(stream));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_timing_enable(hrrt_ctx *ctx, int enable) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    ctx->timing = enable != 0;
+    ctx->ev_used = 0;
+    ctx->ev_marks.clear();
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_timing_read(hrrt_ctx *ctx, double *ms, int *count) {
+    if (!ctx || !ms || !count) return fail(HRRT_ERR_ARG, "null argument");
+    for (int k = 0; k < HRRT_NKERNELS; k++) { ms[k] = 0.0; count[k] = 0; }
+    for (auto &m : ctx->ev_marks) {
+        HIP_TRY(hipEventSynchronize(ctx->ev_pool[m.second + 1]));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, ctx->ev_pool[m.second], ctx->ev_pool[m.second + 1]));
+        ms[m.first] += t;
+        count[m.first] += 1;
+    }
+    ctx->ev_used = 0;
+    ctx->ev_marks.clear();
     return HRRT_OK;
 }
 

@@ -147,6 +147,18 @@ class Osem3D:
             self.end_iteration(image)
         return image
 
+    # ---- kernel timing (HIP events on the launch stream) ----
+    KERNELS = ("fp_ratio", "backproject", "rotate_update")
+
+    def timing_enable(self, on: bool = True):
+        _lib.call("hrrt_timing_enable", self.ctx, int(on))
+
+    def timing_read(self):
+        ms = np.zeros(3, np.float64)
+        cnt = np.zeros(3, np.int32)
+        _lib.call("hrrt_timing_read", self.ctx, ms.ctypes.data_as(C.c_void_p), cnt.ctypes.data_as(C.c_void_p))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.KERNELS)}
+
     # ---- projector interface (H:197-211) ----
     def forward_proj3d(self, image, view: int, prj=None):
         prj = self.estimate_zeros() if prj is None else prj

@@ -83,15 +83,18 @@ def atten_file(def_elements, def_angles, bin_mm, segs, ring_radius, ring_spacing
 
 
 def gaussian_blur_radial(x, sigma_bins):
-    """Blur along the last axis with a normalised Gaussian (zero padding)."""
+    """Blur along the last axis with a normalised Gaussian (zero padding), as a banded matrix product."""
+    n = x.shape[-1]
+    i = torch.arange(n, device=x.device, dtype=torch.float32)
+    k = torch.exp(-0.5 * ((i.view(-1, 1) - i.view(1, -1)) / sigma_bins) ** 2)
     half = int(math.ceil(3 * sigma_bins))
-    k = torch.exp(-0.5 * (torch.arange(-half, half + 1, device=x.device, dtype=torch.float32) / sigma_bins) ** 2)
-    k = (k / k.sum()).view(1, 1, -1)
-    flat = x.reshape(-1, 1, x.shape[-1])
+    k = k * ((i.view(-1, 1) - i.view(1, -1)).abs() <= half)
+    k = k / torch.exp(-0.5 * (torch.arange(-half, half + 1, device=x.device, dtype=torch.float32) / sigma_bins) ** 2).sum()
+    flat = x.reshape(-1, n)
     out = torch.empty_like(flat)
     step = 1 << 20
-    for i in range(0, flat.shape[0], step):
-        out[i:i + step] = torch.nn.functional.conv1d(flat[i:i + step], k, padding=half)
+    for r in range(0, flat.shape[0], step):
+        out[r:r + step] = flat[r:r + step] @ k
     return out.view_as(x)
 
 

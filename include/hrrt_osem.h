@@ -135,6 +135,16 @@ int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subse
                      const int16_t *d_prompt, const float *d_add, int isubset, void *stream);
 int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream);           /* S3:4288-4297           */
 
+/* Kernel timing of hrrt_osem_subset (no counterpart in the reference, whose
+ * only timing is clock() around loops, S3:4219-4285).  When enabled, HIP
+ * events are recorded on the launch stream around the three kernels of every
+ * subset update; hrrt_timing_read waits for them and returns the summed
+ * milliseconds and launch counts per kernel: [0] forward projection + ratio,
+ * [1] back projection, [2] rotation + image update.  Reading resets. */
+#define HRRT_NKERNELS 3
+int hrrt_timing_enable(hrrt_ctx *ctx, int enable);
+int hrrt_timing_read(hrrt_ctx *ctx, double *ms /* [HRRT_NKERNELS] */, int *count /* [HRRT_NKERNELS] */);
+
 /* ---------------- per-view-pair projector calls ---------------- */
 /* forward_proj3d_thread1 / forward_proj3d2 (PJ:1271, 1373): estimate of view
  * pair `view` in the [2S][N] layout.  Bins the reference does not write
