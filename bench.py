@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-subsets", type=int, default=4, help="timed subsets of the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--no-tune", action="store_true", help="skip the forward-projector launch-shape autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -170,6 +171,11 @@ def main():
     image = g.init_image()
     image0 = image.clone()
     order = [int(s) for s in g.order]
+    shape, tune = None, {}
+    if not args.no_tune:
+        from hrrt_reconstruction_amd.autotune import tune_forward
+        shape, tune = tune_forward(g, image, normfac, prompt, add)
+        log("autotune " + ", ".join(f"{k}: {v:.3f} ms" for k, v in tune.items()) + f" -> {shape}")
     for i in range(args.warmup):
         g.osem_subset(image, normfac, prompt, add, order[i % g.subsets])
     torch.cuda.synchronize()
@@ -217,7 +223,8 @@ def main():
         "config": {"workload": "HRRT span 3, 256x256x207 image, 6367-plane 256x288 sinogram, 200M prompts, "
                                "randoms 10%, scatter 15%, 16 subsets (9 view pairs per subset)",
                    "subsets": g.subsets, "view_pairs_per_subset": npairs,
-                   "stage_s": round(stage_s, 3), "normfac_s": round(normfac_s, 3)},
+                   "stage_s": round(stage_s, 3), "normfac_s": round(normfac_s, 3),
+                   "fp_launch_shape": list(shape) if shape else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic},

@@ -1,0 +1,39 @@
+"""Launch-shape autotuner for the forward projector.
+
+The forward projector's workgroup owns the plane blocks of one ray column
+(one x of one view pair); the shape picks how many 4-plane blocks each
+thread accumulates in registers (tasks per thread) and how many workgroups
+share one column (each recomputing the column's rotated samples).  Results
+are bit-identical for every shape; only speed differs.  The tuner times the
+candidates on the caller's own subset update with HIP events and keeps the
+fastest.
+"""
+from __future__ import annotations
+
+CANDIDATES = [(1, 0), (1, 3), (1, 4), (1, 6), (2, 0), (2, 2), (2, 3), (3, 0), (4, 0)]
+
+
+def tune_forward(g, image, normfac, prompt, add, subsets=(0, 8), repeats=2, candidates=CANDIDATES):
+    """Return ((tasks_per_thread, workgroups_per_ray), {shape: ms per forward launch})."""
+    import torch
+    work = image.clone()
+    results = {}
+    for shape in candidates:
+        try:
+            g.set_launch_shape(*shape)
+        except Exception:
+            continue
+        g.osem_subset(work, normfac, prompt, add, subsets[0])      # warm-up / first launch of this variant
+        torch.cuda.synchronize()
+        g.timing_enable(True)
+        for _ in range(repeats):
+            for s in subsets:
+                g.osem_subset(work, normfac, prompt, add, s)
+        kt = g.timing_read()
+        g.timing_enable(False)
+        ms, n = kt["fp_ratio"]
+        results[shape] = ms / max(n, 1)
+    best = min(results, key=results.get)
+    g.set_launch_shape(*best)
+    del work
+    return best, results

@@ -215,9 +215,17 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
             }
         }
     }
-    std::vector<int> tasks;
-    for (int k = 0; k <= G.K; k++)
-        for (int yr = G.yr_bottom[2 * k]; yr <= G.yr_top[2 * k]; yr += 4) tasks.push_back((k << 16) | yr);
+    // forward-projector tasks (4-plane blocks of one segment pair), grouped by code path so
+    // that a wavefront runs one path: axial (theta = 0) blocks, oblique blocks made only of
+    // middle planes, then oblique blocks touching end planes
+    std::vector<int> tasks, t_end;
+    for (int yr = 0; yr <= G.yr_top[0]; yr += 4) tasks.push_back(yr);
+    for (int k = 1; k <= G.K; k++)
+        for (int yr = G.yr_bottom[2 * k]; yr <= G.yr_top[2 * k]; yr += 4) {
+            const bool middle = yr >= G.yr_bottom[2 * k] + ZS && yr + 3 <= G.yr_top[2 * k] - ZS;
+            (middle ? tasks : t_end).push_back((k << 16) | yr);
+        }
+    tasks.insert(tasks.end(), t_end.begin(), t_end.end());
     std::vector<int16_t> pseg(G.spp, -1), plane_seg(G.segmentsize, 0);
     for (int s = 0; s < G.th_pixels; s++)
         for (int i = 0; i < G.nplanes(s); i++) {
@@ -230,6 +238,9 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         file_seg[s] = G.seg_offset[sw] / (N * G.views);
     }
     std::vector<float> resc = G.rescalearray;
+    std::vector<int2> bp_base(G.K + 1, make_int2(0, 0));
+    for (int k = 1; k <= G.K; k++)
+        bp_base[k] = make_int2(G.seg_data[2 * k] - G.yr_bottom[2 * k], G.seg_data[2 * k - 1] - G.yr_bottom[2 * k]);
 
     int r = 0;
     if ((r = ctx->upload(G.cyl_lo, d.cyl_lo)) || (r = ctx->upload(G.cyl_hi, d.cyl_hi)) ||
@@ -240,11 +251,13 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         (r = ctx->upload(resc, d.rescalearray)) || (r = ctx->upload(mid, d.fp_mid)) ||
         (r = ctx->upload(skip, d.fp_skip)) || (r = ctx->upload(endp, d.fp_end)) || (r = ctx->upload(blut, d.bp_lut)) ||
         (r = ctx->upload(rot, d.rot)) || (r = ctx->upload(tasks, d.fp_tasks)) || (r = ctx->upload(pseg, d.pseg)) ||
-        (r = ctx->upload(plane_seg, d.plane_seg))) {
+        (r = ctx->upload(plane_seg, d.plane_seg)) || (r = ctx->upload(bp_base, d.bp_base))) {
         delete ctx;
         return r;
     }
     d.ntasks = (int)tasks.size();
+    d.fp_maxt = 2;
+    d.fp_nsplit = 0;
     {
         std::vector<int8_t> nz((size_t)(G.views / 2) * (c + 1), 1);
         void *p = nullptr;
@@ -365,6 +378,15 @@ extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_no
     ctx->mark(2, S(stream));
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, S(stream)));
     ctx->mark(-1, S(stream));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_set_launch_shape(hrrt_ctx *ctx, int fp_tasks_per_thread, int fp_workgroups_per_ray) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    if (fp_tasks_per_thread < 1 || fp_tasks_per_thread > 4 || fp_workgroups_per_ray < 0 || fp_workgroups_per_ray > 64)
+        return fail(HRRT_ERR_ARG, "launch shape out of range");
+    ctx->dg.fp_maxt = fp_tasks_per_thread;
+    ctx->dg.fp_nsplit = fp_workgroups_per_ray;
     return HRRT_OK;
 }
 

@@ -73,6 +73,7 @@ bool Geometry::build(const hrrt_params &in, std::string &err) {
     z_size = ring_spacing / 2.0f;
     if (views % p.subsets != 0) { err = "views must be a multiple of subsets (PJ:270)"; return false; }
     if (N % 4 != 0 || N < 8) { err = "radial bins must be a multiple of 4"; return false; }
+    if (z_pixels > 256) { err = "more than 128 rings is not supported (z_pixels > 256)"; return false; }
     if (groupmax < 1 || p.span < 2) { err = "need at least one oblique segment group and span >= 2"; return false; }
     sviews = views / p.subsets;
     th_pixels = 2 * groupmax + 1;

@@ -29,7 +29,9 @@ struct DevGeom {
     int ntasks;
     const int16_t *pseg;                 // [spp] padded plane -> segment (-1 on guards)
     const int16_t *plane_seg;            // [S] internal plane -> segment
+    const int2 *bp_base;                 // [K+1]: {seg_data[2k]-yb, seg_data[2k-1]-yb}
     int8_t *nzmask;                      // [views/2][N/2+1], 0 = masked column
+    int fp_maxt, fp_nsplit;              // forward-projector launch shape (tasks/thread, workgroups/(pair,x))
 };
 
 // One batch of view pairs processed together (a subset, or a single pair).

@@ -85,6 +85,58 @@ __device__ __forceinline__ void sample8(const DevGeom &g, const float4 *__restri
     }
 }
 
+// Two-phase version of sample8 for software pipelining: issue the global loads
+// of the next y into registers, project the current y, then combine and store.
+// Each thread owns ITEMS of the 8*NQ (sample, 4-plane chunk) items.
+template <int ITEMS>
+struct SampleRegs {
+    float4 r1[ITEMS], r2[ITEMS], r3[ITEMS], r4[ITEMS];
+    float w1[ITEMS], w2[ITEMS], w3[ITEMS], w4[ITEMS];
+    bool ok[ITEMS];
+};
+
+template <int ITEMS>
+__device__ __forceinline__ void sample8_load(const DevGeom &g, const float4 *__restrict__ image, float cosv,
+                                             float sinv, int x, int y, SampleRegs<ITEMS> &s) {
+    const int NQ = g.NQ, N = g.N, c = g.c;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int it = threadIdx.x + j * blockDim.x;
+        s.ok[j] = false;
+        if (it >= 8 * NQ) continue;
+        const int i = it / NQ, q = it - i * NQ;
+        int xx, yy;
+        sym8(i, x, y, N, xx, yy);
+        const float xx0 = (xx - c) * cosv - (yy - c) * sinv + c;
+        const float yy0 = (xx - c) * sinv + (yy - c) * cosv + c;
+        if (xx0 < 0 || xx0 > N - 2 || yy0 < 0 || xx0 > N - 2) continue;
+        const int xi = (int)xx0, yi = (int)yy0;
+        const float x1 = xx0 - xi, y1 = yy0 - yi;
+        s.w1[j] = (1 - x1) * (1 - y1); s.w2[j] = x1 * (1 - y1); s.w3[j] = y1 * (1 - x1); s.w4[j] = y1 * x1;
+        const float4 *p = image + ((size_t)xi * N + yi) * NQ + q;
+        s.r1[j] = p[0];
+        s.r2[j] = p[(size_t)N * NQ];
+        s.r3[j] = p[NQ];
+        s.r4[j] = p[(size_t)N * NQ + NQ];
+        s.ok[j] = true;
+    }
+}
+
+template <int ITEMS>
+__device__ __forceinline__ void sample8_store(const DevGeom &g, const SampleRegs<ITEMS> &s, float4 *pam, float4 *pam2) {
+    const int NQ = g.NQ;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int it = threadIdx.x + j * blockDim.x;
+        if (it >= 8 * NQ) continue;
+        const int i = it / NQ, q = it - i * NQ;
+        const float4 v = s.ok[j] ? add4(add4(mul4(s.r1[j], s.w1[j]), mul4(s.r2[j], s.w2[j])),
+                                        add4(mul4(s.r3[j], s.w3[j]), mul4(s.r4[j], s.w4[j])))
+                                 : f4(0.f);
+        put_lanes(i < 4 ? pam : pam2, q, i & 3, v, g.Z);
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -97,15 +149,61 @@ __device__ __forceinline__ void sample8(const DevGeom &g, const float4 *__restri
 // tasks of 4 consecutive planes of one segment pair and accumulates them in
 // registers over y in the reference's order.
 // ---------------------------------------------------------------------------
-template <int MAXT, bool RATIO>
+// Write-out of one 4-plane block of one segment for the four lanes
+// (v, x), (v, N-x), (v+90, x), (v+90, N-x) (PJ:1397-1436), optionally through
+// the OP ratio (S3:2493-2505) and the back-projection rescale (PJ:1638-1649).
+template <bool RATIO>
+__device__ __forceinline__ void fp_write(const DevGeom &g, const float4 (&acc)[4], int seg, int yr0, int cnt, int x,
+                                         int view, int lp, float *__restrict__ out,
+                                         const int16_t *__restrict__ prompt, const float *__restrict__ add) {
+    const int N = g.N;
+    const int pidx = g.seg_data[seg] + (yr0 - g.yb[seg]);
+    const float resc = g.bp_rescale[seg];
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        if ((l & 1) && x == g.c) continue;   // at x == N/2 lanes 1/3 duplicate lanes 0/2
+        const int col = (l & 1) ? N - x : x, h = l >> 1;
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = lane(acc[r], l);
+        if (RATIO) {
+            const size_t gi = ((size_t)(view * 2 + h) * N + col) * g.spp + pidx;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (r >= cnt) continue;
+                const float den = v[r] + add[gi + r];
+                const int16_t pr = prompt[gi + r];
+                float q;
+                if (pr == 0) q = 0.0f;
+                else if (den > g.eps3) q = (float)pr / den;
+                else q = 1.0f;
+                v[r] = q * resc;
+            }
+        }
+        float *o = out + ((size_t)(lp * 2 + h) * N + col) * g.spp + pidx;
+        if (cnt == 4) {
+            *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                if (r < cnt) o[r] = v[r];
+        }
+    }
+}
+
+template <int MAXT, int ITEMS, bool RATIO>
 __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatch b,
-                                                 const float4 *__restrict__ image, float *__restrict__ out,
-                                                 const int16_t *__restrict__ prompt, const float *__restrict__ add) {
+                                                  const float4 *__restrict__ image, float *__restrict__ out,
+                                                  const int16_t *__restrict__ prompt, const float *__restrict__ add,
+                                                  int nsplit) {
     extern __shared__ float4 lds[];
-    const int PL = g.Z4 + 4;
+    const int PL = g.Z4 + 8;   // staged planes per column (zero beyond z_pixels)
     float4 *rimc = lds;
     float4 *stage = lds + PL;  // [2][2][PL]
-    const int x = blockIdx.x + 1, lp = blockIdx.y;
+    // longest rays first (x = N/2 has the most y steps), then task split, then view pair
+    const int x = g.c - (int)(blockIdx.x / nsplit), split = blockIdx.x % nsplit, lp = blockIdx.y;
+    const int t_lo = (int)(((long long)g.ntasks * split) / nsplit);
+    const int t_hi = (int)(((long long)g.ntasks * (split + 1)) / nsplit);
     const int view = b.view0 + b.view_stride * lp;
     const int N = g.N, c = g.c, GM = g.GM, ZS = g.ZS;
     if (g.cyl_lo[x] == g.cyl_hi[x]) return;
@@ -131,9 +229,9 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
     int tk[MAXT], ty0[MAXT], tcnt[MAXT];
 #pragma unroll
     for (int t = 0; t < MAXT; t++) {
-        int task = threadIdx.x + t * blockDim.x;
+        int task = t_lo + threadIdx.x + t * blockDim.x;
         tcnt[t] = 0; tk[t] = 0; ty0[t] = 0;
-        if (task < g.ntasks) {
+        if (task < t_hi) {
             int code = g.fp_tasks[task];
             int k = code >> 16, yr0 = code & 0xffff;
             int yt = g.yt[2 * k];
@@ -142,7 +240,7 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
         }
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            float4 v = r < tcnt[t] ? rimc[ty0[t] + r] : f4(0.f);
+            float4 v = rimc[ty0[t] + r];   // planes beyond the segment are never written out
             acc1[t][r] = v;
             acc2[t][r] = v;
         }
@@ -150,16 +248,17 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
 
     for (int y = ystart; y < c; y++) {
         const int st = (y - ystart) & 1;
-        if (y + 1 < c) sample8(g, image, cosv, sinv, x, y + 1, stage + (st ^ 1) * 2 * PL, stage + (st ^ 1) * 2 * PL + PL);
+        SampleRegs<ITEMS> nxt;
+        if (y + 1 < c) sample8_load<ITEMS>(g, image, cosv, sinv, x, y + 1, nxt);
         const float4 *pa = stage + st * 2 * PL, *pb = pa + PL;
 #pragma unroll
         for (int t = 0; t < MAXT; t++) {
             const int cnt = tcnt[t], k = tk[t], yr0 = ty0[t];
             if (cnt == 0) continue;
             if (k == 0) {   // theta = 0 PJ:1907-1910
+                const float4 *qa = pa + yr0, *qb = pb + yr0;
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (r < cnt) acc1[t][r] = add4(acc1[t][r], add4(pa[yr0 + r], pb[yr0 + r]));
+                for (int r = 0; r < 4; r++) acc1[t][r] = add4(acc1[t][r], add4(qa[r], qb[r]));
                 continue;
             }
             const int li = y * GM + k - 1;
@@ -167,20 +266,18 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
             const int yb = g.yb[2 * k], yt = g.yt[2 * k];
             const int4 mid = g.fp_mid[li];
             const float c1 = __int_as_float(mid.z), c2 = __int_as_float(mid.w);
-            if (yr0 >= yb + ZS && yr0 + cnt - 1 <= yt - ZS) {
-                // middle planes PJ:1934-1944: constant fractions, z and z2 advance with yr
+            if (yr0 >= yb + ZS && yr0 + 3 <= yt - ZS) {
+                // four middle planes PJ:1934-1944: constant fractions, z and z2 advance with yr;
+                // a sliding window reuses the upper samples of the previous plane
                 const int z0 = mid.x + (yr0 - (yb + ZS)), z20 = mid.y + (yr0 - (yb + ZS));
-                // sliding window over z: each plane reuses the upper samples of the previous one
-                float4 a0 = pa[z0], b0 = pb[z0], e0 = pa[z20], d0 = pb[z20];
+                const float4 *qa = pa + z0, *qb = pb + z0, *ea = pa + z20, *eb = pb + z20;
+                float4 a0 = qa[0], b0 = qb[0], e0 = ea[0], d0 = eb[0];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    if (r < cnt) {
-                        const float4 a1 = pa[z0 + r + 1], b1 = pb[z0 + r + 1];
-                        const float4 e1 = pa[z20 + r + 1], d1 = pb[z20 + r + 1];
-                        acc1[t][r] = add4(acc1[t][r], add4(mul4(add4(d1, a0), c2), mul4(add4(a1, d0), c1)));
-                        acc2[t][r] = add4(acc2[t][r], add4(mul4(add4(e1, b0), c2), mul4(add4(b1, e0), c1)));
-                        a0 = a1; b0 = b1; e0 = e1; d0 = d1;
-                    }
+                    const float4 a1 = qa[r + 1], b1 = qb[r + 1], e1 = ea[r + 1], d1 = eb[r + 1];
+                    acc1[t][r] = add4(acc1[t][r], add4(mul4(add4(d1, a0), c2), mul4(add4(a1, d0), c1)));
+                    acc2[t][r] = add4(acc2[t][r], add4(mul4(add4(e1, b0), c2), mul4(add4(b1, e0), c1)));
+                    a0 = a1; b0 = b1; e0 = e1; d0 = d1;
                 }
             } else {
 #pragma unroll
@@ -203,6 +300,7 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
                 }
             }
         }
+        if (y + 1 < c) sample8_store<ITEMS>(g, nxt, stage + (st ^ 1) * 2 * PL, stage + (st ^ 1) * 2 * PL + PL);
         __syncthreads();
     }
 
@@ -213,52 +311,12 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
         if (cnt == 0) continue;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            if (r >= cnt) continue;
-            if (k == 0) {
-                acc1[t][r] = mul4(acc1[t][r], g.y_size);
-            } else {
-                const float s = g.rescalearray[(k - 1) * g.Z + yr0 + r];
-                acc1[t][r] = mul4(acc1[t][r], s);
-                acc2[t][r] = mul4(acc2[t][r], s);
-            }
+            const float sc = k == 0 ? g.y_size : g.rescalearray[(k - 1) * g.Z + min(yr0 + r, g.Z - 1)];
+            acc1[t][r] = mul4(acc1[t][r], sc);
+            acc2[t][r] = mul4(acc2[t][r], sc);
         }
-#pragma unroll
-        for (int si = 0; si < 2; si++) {
-            if (si == 1 && k == 0) continue;
-            const int seg = k == 0 ? 0 : (si == 0 ? 2 * k : 2 * k - 1);
-            const int pidx = g.seg_data[seg] + (yr0 - g.yb[seg]);
-            const float resc = g.bp_rescale[seg];
-#pragma unroll
-            for (int l = 0; l < 4; l++) {
-                if ((l & 1) && x == c) continue;   // at x == N/2 lanes 1/3 duplicate lanes 0/2
-                const int col = (l & 1) ? N - x : x, h = l >> 1;
-                float v[4];
-#pragma unroll
-                for (int r = 0; r < 4; r++) v[r] = lane(si == 0 ? acc1[t][r] : acc2[t][r], l);
-                if (RATIO) {
-                    const size_t gi = ((size_t)(view * 2 + h) * N + col) * g.spp + pidx;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        if (r >= cnt) continue;
-                        const float den = v[r] + add[gi + r];
-                        const int16_t pr = prompt[gi + r];
-                        float q;
-                        if (pr == 0) q = 0.0f;
-                        else if (den > g.eps3) q = (float)pr / den;
-                        else q = 1.0f;
-                        v[r] = q * resc;
-                    }
-                }
-                float *o = out + ((size_t)(lp * 2 + h) * N + col) * g.spp + pidx;
-                if (cnt == 4) {
-                    *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
-                } else {
-#pragma unroll
-                    for (int r = 0; r < 3; r++)
-                        if (r < cnt) o[r] = v[r];
-                }
-            }
-        }
+        fp_write<RATIO>(g, acc1[t], k == 0 ? 0 : 2 * k, yr0, cnt, x, view, lp, out, prompt, add);
+        if (k != 0) fp_write<RATIO>(g, acc2[t], 2 * k - 1, yr0, cnt, x, view, lp, out, prompt, add);
     }
 }
 
@@ -294,7 +352,9 @@ __global__ void __launch_bounds__(256) bp_kernel(const DevGeom g, const PairBatc
     const bool masked = g.nzmask[view * (c + 1) + mcol] == 0;
     float *dst = role == 0 ? bufA : bufB;
     dst += (size_t)lp * N * N * g.Z4;
-    const int wave = threadIdx.x >> 6, lanei = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int lanei = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr int NS = 4;   // z slots per lane: z = lane + 64*j covers Z <= 256
     for (int V = lo + wave; V < hi; V += nw) {
         const int X = role == 0 ? col : V, Y = role == 0 ? V : col;
         float *o = dst + ((size_t)X * N + Y) * g.Z4;
@@ -304,28 +364,38 @@ __global__ void __launch_bounds__(256) bp_kernel(const DevGeom g, const PairBatc
         }
         const int sl = V <= c ? V : N - V, ll = N - sl;
         const bool first_even = role == 0 ? (V <= c) : (V >= c);
-        for (int z = lanei; z < Z; z += 64) {
-            const float t0 = colv[g.seg_data[0] + z];
-            float i1 = first_even ? t0 : 0.0f;
-            float i2 = first_even ? 0.0f : t0;
-            for (int k = 1; k <= g.K; k++) {
-                const int4 L1 = g.bp_lut[sl * GM + k - 1];
-                const int4 L2 = g.bp_lut[ll * GM + k - 1];
-                const int yb = g.yb[2 * k];
-                const int be = g.seg_data[2 * k] - yb, bo = g.seg_data[2 * k - 1] - yb;
-                const int b1 = first_even ? be : bo, b2 = first_even ? bo : be;
+        float i1[NS], i2[NS];
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            const int z = lanei + 64 * j;
+            const float t0 = z < Z ? colv[g.seg_data[0] + z] : 0.0f;
+            i1[j] = first_even ? t0 : 0.0f;
+            i2[j] = first_even ? 0.0f : t0;
+        }
+        const int4 *lut1 = g.bp_lut + sl * GM, *lut2 = g.bp_lut + ll * GM;
+        for (int k = 1; k <= g.K; k++) {
+            const int4 L1 = lut1[k - 1], L2 = lut2[k - 1];       // wave-uniform (scalar) loads
+            const int2 bb = g.bp_base[k];
+            const int b1 = (first_even ? bb.x : bb.y) + L1.z, b2 = (first_even ? bb.y : bb.x) + L2.z;
+            const float f1 = __int_as_float(L1.w), f2 = __int_as_float(L2.w);
+            const int e2 = L2.x + (L1.y - L1.x);
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                const int z = lanei + 64 * j;
                 if (z >= L1.x && z <= L1.y) {
-                    const int yr = z + L1.z;
-                    const float p = colv[b1 + yr], p1 = colv[b1 + yr + 1];
-                    i1 = i1 + (p + __int_as_float(L1.w) * (p1 - p));
+                    const float p = colv[b1 + z], p1 = colv[b1 + z + 1];
+                    i1[j] = i1[j] + (p + f1 * (p1 - p));
                 }
-                if (z >= L2.x && z <= L2.x + (L1.y - L1.x)) {
-                    const int yr = z + L2.z;
-                    const float p = colv[b2 + yr], p1 = colv[b2 + yr + 1];
-                    i2 = i2 + (p + __int_as_float(L2.w) * (p1 - p));
+                if (z >= L2.x && z <= e2) {
+                    const float p = colv[b2 + z], p1 = colv[b2 + z + 1];
+                    i2[j] = i2[j] + (p + f2 * (p1 - p));
                 }
             }
-            o[z] = i1 + i2;
+        }
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            const int z = lanei + 64 * j;
+            if (z < Z) o[z] = i1[j] + i2[j];
         }
     }
 }
@@ -642,31 +712,52 @@ __global__ void nzmask_kernel(const DevGeom g, const int *__restrict__ colmask, 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int MAXT>
+template <int MAXT, int ITEMS>
 static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float *image, float *out,
-                              const int16_t *prompt, const float *add, int ratio, hipStream_t s) {
-    int threads = (g.ntasks + MAXT - 1) / MAXT;
-    threads = ((threads + 63) / 64) * 64;
-    if (threads > 512) return hipErrorInvalidConfiguration;
-    dim3 grid(g.c, b.npairs), block(threads);
-    size_t shm = sizeof(float4) * 5 * (g.Z4 + 4);
+                              const int16_t *prompt, const float *add, int ratio, int nsplit, int threads,
+                              hipStream_t s) {
+    dim3 grid(g.c * nsplit, b.npairs), block(threads);
+    size_t shm = sizeof(float4) * 5 * (g.Z4 + 8);
     if (ratio)
-        hipLaunchKernelGGL((fp_kernel<MAXT, true>), grid, block, shm, s, g, b,
-                           reinterpret_cast<const float4 *>(image), out, prompt, add);
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, true>), grid, block, shm, s, g, b,
+                           reinterpret_cast<const float4 *>(image), out, prompt, add, nsplit);
     else
-        hipLaunchKernelGGL((fp_kernel<MAXT, false>), grid, block, shm, s, g, b,
-                           reinterpret_cast<const float4 *>(image), out, prompt, add);
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, false>), grid, block, shm, s, g, b,
+                           reinterpret_cast<const float4 *>(image), out, prompt, add, nsplit);
     return hipGetLastError();
 }
 
 hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *image, float *out,
                           const int16_t *prompt, const float *add, int ratio_mode, hipStream_t s) {
-    // choose the smallest task count per thread that keeps the workgroup within 512 threads
-    if ((g.ntasks + 0) <= 512 * 1 && g.ntasks <= 256) return fp_launch_t<1>(g, b, image, out, prompt, add, ratio_mode, s);
-    if (g.ntasks <= 2 * 256) return fp_launch_t<2>(g, b, image, out, prompt, add, ratio_mode, s);
-    if (g.ntasks <= 3 * 320) return fp_launch_t<3>(g, b, image, out, prompt, add, ratio_mode, s);
-    if (g.ntasks <= 4 * 512) return fp_launch_t<4>(g, b, image, out, prompt, add, ratio_mode, s);
-    return hipErrorInvalidConfiguration;
+    // Launch shape: MAXT tasks (4-plane blocks) per thread; the tasks of one ray column
+    // are split over nsplit workgroups; each workgroup has just enough 64-lane waves
+    // (at most 8) for its share of the tasks.
+    int maxt = g.fp_maxt, nsplit = g.fp_nsplit;
+    if (maxt < 1 || maxt > 4) maxt = 1;
+    const int need = (g.ntasks + 512 * maxt - 1) / (512 * maxt);
+    if (nsplit < need) nsplit = need;
+    const int per = (g.ntasks + nsplit - 1) / nsplit;
+    int threads = (((per + maxt - 1) / maxt) + 63) / 64 * 64;
+    if (threads < 64) threads = 64;
+    const int items = (8 * g.NQ + threads - 1) / threads;
+    if (items > 2) {
+        // small workgroups: use enough threads for the sampling phase
+        threads = (8 * g.NQ + 1) / 2;
+        threads = (threads + 63) / 64 * 64;
+    }
+    const int it2 = (8 * g.NQ + threads - 1) / threads;
+#define FP_CASE(M)                                                                                    \
+    case M:                                                                                           \
+        return it2 <= 1 ? fp_launch_t<M, 1>(g, b, image, out, prompt, add, ratio_mode, nsplit, threads, s) \
+                        : fp_launch_t<M, 2>(g, b, image, out, prompt, add, ratio_mode, nsplit, threads, s);
+    switch (maxt) {
+        FP_CASE(1)
+        FP_CASE(2)
+        FP_CASE(3)
+        default:
+            FP_CASE(4)
+    }
+#undef FP_CASE
 }
 
 hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const float *src, int src_global,

@@ -150,6 +150,9 @@ class Osem3D:
     # ---- kernel timing (HIP events on the launch stream) ----
     KERNELS = ("fp_ratio", "backproject", "rotate_update")
 
+    def set_launch_shape(self, tasks_per_thread: int, workgroups_per_ray: int = 0):
+        _lib.call("hrrt_set_launch_shape", self.ctx, int(tasks_per_thread), int(workgroups_per_ray))
+
     def timing_enable(self, on: bool = True):
         _lib.call("hrrt_timing_enable", self.ctx, int(on))
 

@@ -142,6 +142,12 @@ int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream);           /
  * milliseconds and launch counts per kernel: [0] forward projection + ratio,
  * [1] back projection, [2] rotation + image update.  Reading resets. */
 #define HRRT_NKERNELS 3
+/* Launch shape of the forward projector (no counterpart in the reference; set
+ * by the autotuner): each workgroup of 256 threads owns the plane blocks of
+ * one ray column split `fp_workgroups_per_ray` ways (0 = as few as fit), with
+ * `fp_tasks_per_thread` (1..4) blocks of 4 planes per thread.  Results do not
+ * depend on the shape. */
+int hrrt_set_launch_shape(hrrt_ctx *ctx, int fp_tasks_per_thread, int fp_workgroups_per_ray);
 int hrrt_timing_enable(hrrt_ctx *ctx, int enable);
 int hrrt_timing_read(hrrt_ctx *ctx, double *ms /* [HRRT_NKERNELS] */, int *count /* [HRRT_NKERNELS] */);
 
