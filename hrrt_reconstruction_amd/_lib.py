@@ -50,6 +50,7 @@ SIGNATURES = {
     "hrrt_get_dims": (_I, [_VP, C.POINTER(Dims)]),
     "hrrt_get_order": (_I, [_VP, _VP, _VP]),
     "hrrt_get_segments": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "hrrt_query_geometry": (_I, [C.POINTER(Params), C.POINTER(Dims)] + [_VP] * 6),
     "hrrt_set_norm_zero_mask": (_I, [_VP, _VP]),
     "hrrt_get_norm_zero_mask": (_I, [_VP, _VP]),
     "hrrt_last_error": (C.c_char_p, []),

@@ -277,9 +277,7 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
 
 extern "C" void hrrt_destroy(hrrt_ctx *ctx) { delete ctx; }
 
-extern "C" int hrrt_get_dims(const hrrt_ctx *ctx, hrrt_dims *o) {
-    if (!ctx || !o) return fail(HRRT_ERR_ARG, "null argument");
-    const hrrt::Geometry &G = ctx->geo;
+static void fill_dims(const hrrt::Geometry &G, hrrt_dims *o) {
     o->N = G.N; o->views = G.views; o->z_pixels = G.z_pixels; o->z4 = G.z4; o->segmentsize = G.segmentsize;
     o->th_pixels = G.th_pixels; o->sviews = G.sviews; o->subsets = G.p.subsets; o->spp = G.spp;
     o->x_rebin = G.x_rebin; o->v_rebin = G.v_rebin;
@@ -287,24 +285,48 @@ extern "C" int hrrt_get_dims(const hrrt_ctx *ctx, hrrt_dims *o) {
     o->sino_bins = (long long)G.segmentsize * G.views * G.N;
     o->file_bins = (long long)G.segmentsize * G.p.def_angles * G.p.def_elements;
     o->paircol_elems = (long long)(G.views / 2) * 2 * G.N * G.spp;
-    return HRRT_OK;
 }
 
-extern "C" int hrrt_get_segments(const hrrt_ctx *ctx, int *yb, int *yt, int *so, int *sd) {
-    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
-    const hrrt::Geometry &G = ctx->geo;
+static void copy_segments(const hrrt::Geometry &G, int *yb, int *yt, int *so, int *sd) {
     const size_t n = sizeof(int) * G.th_pixels;
     if (yb) std::memcpy(yb, G.yr_bottom.data(), n);
     if (yt) std::memcpy(yt, G.yr_top.data(), n);
     if (so) std::memcpy(so, G.segment_offset.data(), n);
     if (sd) std::memcpy(sd, G.seg_data.data(), n);
+}
+
+static void copy_order(const hrrt::Geometry &G, int *order, int *vieworder) {
+    if (order) std::memcpy(order, G.order.data(), sizeof(int) * G.order.size());
+    if (vieworder) std::memcpy(vieworder, G.vieworder.data(), sizeof(int) * G.vieworder.size());
+}
+
+extern "C" int hrrt_query_geometry(const hrrt_params *params, hrrt_dims *dims, int *order, int *vieworder,
+                                   int *yb, int *yt, int *so, int *sd) {
+    if (!params) return fail(HRRT_ERR_ARG, "null argument");
+    hrrt::Geometry G;
+    std::string err;
+    if (!G.build(*params, err)) return fail(HRRT_ERR_ARG, "%s", err.c_str());
+    if (dims) fill_dims(G, dims);
+    copy_order(G, order, vieworder);
+    copy_segments(G, yb, yt, so, sd);
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_get_dims(const hrrt_ctx *ctx, hrrt_dims *o) {
+    if (!ctx || !o) return fail(HRRT_ERR_ARG, "null argument");
+    fill_dims(ctx->geo, o);
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_get_segments(const hrrt_ctx *ctx, int *yb, int *yt, int *so, int *sd) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    copy_segments(ctx->geo, yb, yt, so, sd);
     return HRRT_OK;
 }
 
 extern "C" int hrrt_get_order(const hrrt_ctx *ctx, int *order, int *vieworder) {
     if (!ctx) return fail(HRRT_ERR_ARG, "null context");
-    if (order) std::memcpy(order, ctx->geo.order.data(), sizeof(int) * ctx->geo.order.size());
-    if (vieworder) std::memcpy(vieworder, ctx->geo.vieworder.data(), sizeof(int) * ctx->geo.vieworder.size());
+    copy_order(ctx->geo, order, vieworder);
     return HRRT_OK;
 }
 

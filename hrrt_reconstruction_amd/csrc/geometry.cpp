@@ -234,6 +234,36 @@ bool Geometry::build(const hrrt_params &in, std::string &err) {
         }
     }
 
+    // incremental back-projection breakpoints PJ:718-784: walking y away from the
+    // centre row, a breakpoint is a row where floor((c-y)*tan) changes; it records
+    // the lookup of that row and of the previous breakpoint row
+    blut.assign(static_cast<size_t>(N) * groupmax, Blut{0, 0, 0, 0, 0, 0, 0.f, 0.f, -1});
+    blut_flag.assign(static_cast<size_t>(N) * groupmax, 0);
+    for (int k = 0; k < groupmax; k++) {
+        const int c = N / 2;
+        auto L = [&](int y) { return static_cast<size_t>(y) * groupmax + k; };
+        blut_flag[L(c)] = 1;
+        blut[L(c)] = Blut{bl_zsa[L(c)], bl_yra[L(c)], bl_zea[L(c)], 0, 0, 0, bl_frac[L(c)], 0.f, 0};
+        for (int dir = 0; dir < 2; dir++) {
+            int prev = 0, cnt = 0, grp = c;
+            if (dir == 1) blut_flag[L(c)] = 0;
+            for (int y = dir == 0 ? c + 1 : c - 1; dir == 0 ? y < N : y >= 0; y += dir == 0 ? 1 : -1) {
+                blut_flag[L(y)] = 0;
+                const int dd = bl_yra[L(y)] - bl_zsa[L(y)];
+                if (prev == dd) {
+                    cnt++;
+                } else {
+                    blut_flag[L(y)] = cnt;
+                    blut[L(y)] = Blut{bl_zsa[L(y)], bl_yra[L(y)], bl_zea[L(y)], bl_zsa[L(grp)], bl_yra[L(grp)],
+                                      bl_zea[L(grp)], bl_frac[L(y)], bl_frac[L(grp)], cnt};
+                    grp = y;
+                    cnt = 0;
+                    prev = dd;
+                }
+            }
+        }
+    }
+
     // padded pair-column layout: each segment's planes start on a multiple of 4
     // with one zero guard plane on each side
     seg_data.assign(th_pixels, 0);

@@ -35,6 +35,10 @@ struct Geometry {
     // backward lookup [N][groupmax]
     std::vector<int> bl_zsa, bl_zea, bl_yra;
     std::vector<float> bl_frac;
+    // incremental back-projection breakpoints [N][groupmax] (PJ:718-784)
+    struct Blut { int z_start, yr_start, z_end, z_start_pre, yr_start_pre, z_end_pre; float coef, coef_pre; int num; };
+    std::vector<Blut> blut;
+    std::vector<int> blut_flag;
     // subsets
     std::vector<int> vieworder;   // [subsets][sviews]
     std::vector<int> order;       // [subsets]

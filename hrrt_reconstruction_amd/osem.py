@@ -39,29 +39,28 @@ def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-class Osem3D:
-    """One reconstruction context (geometry tables resident on the device)."""
+class Layout:
+    """Dimensions, segment tables and subset order of a geometry (host only, no device).
 
-    def __init__(self, device: str = "cuda:0", **params):
-        self.device = torch.device(device)
-        torch.cuda.set_device(self.device)
-        self.params = make_params(**params)
-        h = C.c_void_p()
-        _lib.call("hrrt_create", C.byref(self.params), C.byref(h))
-        self.ctx = h
+    Computed by the library's own geometry code (hrrt_query_geometry), so the
+    values are the ones a context on the device uses.
+    """
+
+    def __init__(self, params: _lib.Params):
+        self.params = params
         d = _lib.Dims()
-        _lib.call("hrrt_get_dims", self.ctx, C.byref(d))
+        _lib.call("hrrt_query_geometry", C.byref(params), C.byref(d), *([None] * 6))
         self.dims = d
         self.N, self.views, self.Z, self.Z4 = d.N, d.views, d.z_pixels, d.z4
         self.S, self.spp, self.subsets, self.sviews = d.segmentsize, d.spp, d.subsets, d.sviews
+        self.th = d.th_pixels
         order = np.zeros(self.subsets, np.int32)
         vo = np.zeros(self.subsets * self.sviews, np.int32)
-        _lib.call("hrrt_get_order", self.ctx, order.ctypes.data_as(C.c_void_p), vo.ctypes.data_as(C.c_void_p))
+        seg = [np.zeros(self.th, np.int32) for _ in range(4)]
+        ptrs = [a.ctypes.data_as(C.c_void_p) for a in [order, vo] + seg]
+        _lib.call("hrrt_query_geometry", C.byref(params), None, *ptrs)
         self.order = order
         self.vieworder = vo.reshape(self.subsets, self.sviews)
-        th = d.th_pixels
-        seg = [np.zeros(th, np.int32) for _ in range(4)]
-        _lib.call("hrrt_get_segments", self.ctx, *[a.ctypes.data_as(C.c_void_p) for a in seg])
         self.yr_bottom, self.yr_top, self.segment_offset, self.seg_data = seg
 
     def paircol_index(self):
@@ -81,6 +80,26 @@ class Osem3D:
         s2 = sino.reshape(self.S, 2, V // 2, N)          # view = h*V/2 + pair
         out[:, :, :, pidx] = np.transpose(s2[planes], (2, 1, 3, 0))
         return out
+
+    def paircol_to_sino(self, pc):
+        """Inverse of sino_to_paircol: [views/2][2][N][spp] -> [S][views][N] (host numpy)."""
+        planes, pidx = self.paircol_index()
+        V, N = self.views, self.N
+        out = np.zeros((self.S, 2, V // 2, N), pc.dtype)
+        out[planes] = np.transpose(pc[:, :, :, pidx], (3, 1, 0, 2))
+        return out.reshape(self.S, V, N)
+
+
+class Osem3D(Layout):
+    """One reconstruction context (geometry tables resident on the device)."""
+
+    def __init__(self, device: str = "cuda:0", **params):
+        super().__init__(make_params(**params))
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        h = C.c_void_p()
+        _lib.call("hrrt_create", C.byref(self.params), C.byref(h))
+        self.ctx = h
 
     def close(self):
         if getattr(self, "ctx", None):

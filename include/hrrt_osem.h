@@ -94,6 +94,12 @@ int  hrrt_get_dims(const hrrt_ctx *ctx, hrrt_dims *out);
 int  hrrt_get_segments(const hrrt_ctx *ctx, int *yr_bottom, int *yr_top, int *segment_offset, int *seg_data);
 /* Subset order (sequence S3:535) and view order (calculatevieworder PJ:144). */
 int  hrrt_get_order(const hrrt_ctx *ctx, int *order /* [subsets] */, int *vieworder /* [subsets][sviews] */);
+/* The same dimensions and tables as hrrt_get_dims/_order/_segments, computed
+ * on the host without a context or a device, so that a caller can size its
+ * buffers before it picks a GPU.  Any output pointer may be NULL; the table
+ * outputs need th_pixels / subsets / subsets*sviews ints (read dims first). */
+int  hrrt_query_geometry(const hrrt_params *params, hrrt_dims *dims, int *order, int *vieworder,
+                         int *yr_bottom, int *yr_top, int *segment_offset, int *seg_data);
 /* norm_zero_mask (S3:864-909): [views/2][N/2+1] int8, 0 = the (v, v+views/2, x, N-x)
  * columns of the norm are zero in every plane.  Staging computes it; a driver
  * that prepares the norm itself sets it here. */
