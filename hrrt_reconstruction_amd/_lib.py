@@ -61,6 +61,7 @@ SIGNATURES = {
     "hrrt_end_iteration": (_I, [_VP, _VP, _VP]),
     "hrrt_timing_enable": (_I, [_VP, _I]),
     "hrrt_set_launch_shape": (_I, [_VP, _I, _I]),
+    "hrrt_set_backproject_shape": (_I, [_VP, _I]),
     "hrrt_timing_read": (_I, [_VP, _VP, _VP]),
     "hrrt_forward_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_back_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),

@@ -1,16 +1,48 @@
-"""Launch-shape autotuner for the forward projector.
+"""Launch-shape autotuner for the projectors.
 
 The forward projector's workgroup owns the plane blocks of one ray column
 (one x of one view pair); the shape picks how many 4-plane blocks each
 thread accumulates in registers (tasks per thread) and how many workgroups
-share one column (each recomputing the column's rotated samples).  Results
-are bit-identical for every shape; only speed differs.  The tuner times the
+share one column (each recomputing the column's rotated samples).  The back
+projector's workgroup stages one sinogram column in LDS; its shape is the
+number of waves (voxel rows in flight) sharing that column.  Results are
+bit-identical for every shape; only speed differs.  The tuner times the
 candidates on the caller's own subset update with HIP events and keeps the
 fastest.
 """
 from __future__ import annotations
 
 CANDIDATES = [(1, 0), (1, 3), (1, 4), (1, 6), (2, 0), (2, 2), (2, 3), (3, 0), (4, 0)]
+
+
+BP_CANDIDATES = [4, 8, 12, 16]
+
+
+def _time_kernel(g, work, normfac, prompt, add, subsets, repeats, kernel):
+    import torch
+    g.osem_subset(work, normfac, prompt, add, subsets[0])      # warm-up / first launch of this variant
+    torch.cuda.synchronize()
+    g.timing_enable(True)
+    for _ in range(repeats):
+        for s in subsets:
+            g.osem_subset(work, normfac, prompt, add, s)
+    kt = g.timing_read()
+    g.timing_enable(False)
+    ms, n = kt[kernel]
+    return ms / max(n, 1)
+
+
+def tune_backproject(g, image, normfac, prompt, add, subsets=(0, 8), repeats=2, candidates=BP_CANDIDATES):
+    """Return (waves_per_workgroup, {waves: ms per back-projection launch})."""
+    work = image.clone()
+    results = {}
+    for w in candidates:
+        g.set_backproject_shape(w)
+        results[w] = _time_kernel(g, work, normfac, prompt, add, subsets, repeats, "backproject")
+    best = min(results, key=results.get)
+    g.set_backproject_shape(best)
+    del work
+    return best, results
 
 
 def tune_forward(g, image, normfac, prompt, add, subsets=(0, 8), repeats=2, candidates=CANDIDATES):

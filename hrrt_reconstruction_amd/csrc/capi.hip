@@ -238,9 +238,62 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         file_seg[s] = G.seg_offset[sw] / (N * G.views);
     }
     std::vector<float> resc = G.rescalearray;
-    std::vector<int2> bp_base(G.K + 1, make_int2(0, 0));
+    // back-projection column in LDS: segment 0, then every oblique segment inside a window of
+    // zero planes wide enough that the back projector's 64-plane slots 0-2 (z < 192) always,
+    // and slot 3 when the range reaches z >= 192, read only that segment or zeros (the
+    // reference's zero guard planes PJ:1689-1692, extended); neighbouring windows share zeros
+    if (GM > 32) {
+        delete ctx;
+        return fail(HRRT_ERR_UNSUPPORTED, "more than 32 oblique segment groups");
+    }
+    std::vector<int> need_lo(G.th_pixels, 0), need_hi(G.th_pixels, 0);
     for (int k = 1; k <= G.K; k++)
-        bp_base[k] = make_int2(G.seg_data[2 * k] - G.yr_bottom[2 * k], G.seg_data[2 * k - 1] - G.yr_bottom[2 * k]);
+        for (int y = 1; y < N; y++) {
+            const int4 L = blut[(size_t)y * GM + k - 1];
+            const int yb = G.yr_bottom[2 * k], yt = G.yr_top[2 * k];
+            int top = 192 + L.z;                          // slots 0-2 read yr = z + off, z+1 <= 192
+            if (L.y >= 192) top = std::max(top, 256 + L.z);
+            for (int s = 2 * k - 1; s <= 2 * k; s++) {
+                need_lo[s] = std::max(need_lo[s], yb - L.z);
+                need_hi[s] = std::max(need_hi[s], top - yt);
+            }
+        }
+    std::vector<int> padd(G.th_pixels, 0);
+    int lds = G.nplanes(0);
+    for (int s = 1; s < G.th_pixels; s++) {
+        padd[s] = lds + std::max(need_lo[s], s > 1 ? need_hi[s - 1] : 0);
+        lds = padd[s] + G.nplanes(s);
+    }
+    lds += std::max(need_hi[G.th_pixels - 1], 1);
+    if ((size_t)lds * sizeof(float2) > 160 * 1024) {
+        delete ctx;
+        return fail(HRRT_ERR_UNSUPPORTED, "back-projection column (%d planes) exceeds LDS", lds);
+    }
+    d.bp_lds = lds;
+    std::vector<int> bp_src(lds + 1, -1);
+    for (int s = 0; s < G.th_pixels; s++)
+        for (int i = 0; i < G.nplanes(s); i++) bp_src[padd[s] + i] = G.seg_data[s] + i;
+    // per (row, group): {zsa | zea << 16, LDS index of plane 0 of segment 2k shifted by the
+    // row's integer offset, same for 2k-1, bits(frac)}; every read must stay in its window
+    std::vector<int4> blut2((size_t)N * GM);
+    for (int y = 0; y < N; y++)
+        for (int k = 1; k <= G.K; k++) {
+            const int4 L = blut[(size_t)y * GM + k - 1];
+            const int yb = G.yr_bottom[2 * k];
+            blut2[(size_t)y * GM + k - 1] =
+                make_int4(L.x | (L.y << 16), padd[2 * k] - yb + L.z, padd[2 * k - 1] - yb + L.z, L.w);
+            if (y == 0) continue;
+            for (int s = 2 * k - 1; s <= 2 * k; s++) {
+                const int b = padd[s] - yb + L.z;
+                const int last = (L.y >= 192 ? 256 : 192);          // last plane touched: z = last-1, p[z+1]
+                const int wlo = padd[s] - need_lo[s], whi = padd[s] + G.nplanes(s) + need_hi[s];  // [wlo, whi)
+                if (b < wlo || b + last >= whi || b + last >= lds) {
+                    delete ctx;
+                    return fail(HRRT_ERR_UNSUPPORTED, "back-projection read outside its LDS window (y=%d, group=%d)",
+                                y, k);
+                }
+            }
+        }
 
     int r = 0;
     if ((r = ctx->upload(G.cyl_lo, d.cyl_lo)) || (r = ctx->upload(G.cyl_hi, d.cyl_hi)) ||
@@ -251,13 +304,15 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         (r = ctx->upload(resc, d.rescalearray)) || (r = ctx->upload(mid, d.fp_mid)) ||
         (r = ctx->upload(skip, d.fp_skip)) || (r = ctx->upload(endp, d.fp_end)) || (r = ctx->upload(blut, d.bp_lut)) ||
         (r = ctx->upload(rot, d.rot)) || (r = ctx->upload(tasks, d.fp_tasks)) || (r = ctx->upload(pseg, d.pseg)) ||
-        (r = ctx->upload(plane_seg, d.plane_seg)) || (r = ctx->upload(bp_base, d.bp_base))) {
+        (r = ctx->upload(plane_seg, d.plane_seg)) || (r = ctx->upload(bp_src, d.bp_src)) ||
+        (r = ctx->upload(blut2, d.bp_lut2))) {
         delete ctx;
         return r;
     }
     d.ntasks = (int)tasks.size();
     d.fp_maxt = 2;
     d.fp_nsplit = 0;
+    d.bp_waves = 8;
     {
         std::vector<int8_t> nz((size_t)(G.views / 2) * (c + 1), 1);
         void *p = nullptr;
@@ -409,6 +464,13 @@ extern "C" int hrrt_set_launch_shape(hrrt_ctx *ctx, int fp_tasks_per_thread, int
         return fail(HRRT_ERR_ARG, "launch shape out of range");
     ctx->dg.fp_maxt = fp_tasks_per_thread;
     ctx->dg.fp_nsplit = fp_workgroups_per_ray;
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_set_backproject_shape(hrrt_ctx *ctx, int waves_per_workgroup) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    if (waves_per_workgroup < 1 || waves_per_workgroup > 16) return fail(HRRT_ERR_ARG, "launch shape out of range");
+    ctx->dg.bp_waves = waves_per_workgroup;
     return HRRT_OK;
 }
 

@@ -24,14 +24,18 @@ struct DevGeom {
     const int *fp_skip;                  // [N][GM]
     const int4 *fp_end;                  // [N][GM][2ZS]: {z, bits(f1), z2, bits(f2)}
     const int4 *bp_lut;                  // [N][GM]: {zsa, zea, off = yra - zsa, bits(frac)}
+    const int4 *bp_lut2;                 // [N][GM]: {zsa | zea << 16, LDS index of plane 0 of the shifted
+                                         //  segment 2k, of segment 2k-1, bits(frac)}
     const float2 *rot;                   // [views/2][N][N] rotation sample positions (rotateimage2 order)
     const int *fp_tasks;                 // [ntasks]: (k << 16) | yr0
     int ntasks;
     const int16_t *pseg;                 // [spp] padded plane -> segment (-1 on guards)
     const int16_t *plane_seg;            // [S] internal plane -> segment
-    const int2 *bp_base;                 // [K+1]: {seg_data[2k]-yb, seg_data[2k-1]-yb}
+    const int *bp_src;                   // [bp_lds + 1] LDS entry -> pair-column plane (-1 = zero)
+    int bp_lds;                          // entries of the zero-padded back-projection column
     int8_t *nzmask;                      // [views/2][N/2+1], 0 = masked column
     int fp_maxt, fp_nsplit;              // forward-projector launch shape (tasks/thread, workgroups/(pair,x))
+    int bp_waves;                        // back-projector waves per workgroup
 };
 
 // One batch of view pairs processed together (a subset, or a single pair).

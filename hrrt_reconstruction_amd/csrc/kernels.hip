@@ -325,27 +325,37 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
 // back_proj3d_view_n1 PJ:2927-3086), as a gather.  For a rotated-frame voxel
 // (X, Y) the reference's pass over view v gives A(X,Y) from sinogram column X
 // and its pass over view v+90 gives B(X,Y) from column Y; R = A + B.  One
-// workgroup per (column, role, view pair) stages the padded column in LDS and
-// evaluates every voxel of the disk row/column that reads it.
+// workgroup per (column, role, view pair) stages the column in LDS and
+// evaluates every voxel of the disk row/column that reads it; a wave owns one
+// voxel and its lanes the planes z = lane + 64 j.
+//
+// LDS holds (p[i], p[i+1] - p[i]) per plane, the difference rounded as the
+// reference rounds it, and every oblique segment sits inside a window of zero
+// planes sized on the host from its rows' shifts.  Slots 0-2 (z < 192) are
+// therefore evaluated without any range test: lanes outside the reference's
+// range [zsa, zea] (PJ:2986-2995) interpolate between zeros and add exactly +0;
+// slot 3 runs only when the range reaches it (a wave-uniform test).  The wave
+// keeps its voxel row's lookup entries in VGPRs (lane k-1: row sl, lane 32+k-1:
+// row N-sl) and takes entry k with readlane, so the loop over segment groups
+// waits on nothing but its own LDS reads.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) bp_kernel(const DevGeom g, const PairBatch b, const float *__restrict__ src,
-                                                 int src_global, int apply_rescale, const int16_t *__restrict__ pseg,
-                                                 float *__restrict__ bufA, float *__restrict__ bufB) {
-    extern __shared__ float colv[];
+__global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBatch b, const float *__restrict__ src,
+                                                  int src_global, int apply_rescale, const int16_t *__restrict__ pseg,
+                                                  float *__restrict__ bufA, float *__restrict__ bufB) {
+    extern __shared__ float2 colpq[];
     const int col = blockIdx.x + 1, role = blockIdx.y, lp = blockIdx.z;
     const int view = b.view0 + b.view_stride * lp;
-    const int N = g.N, c = g.c, Z = g.Z, spp = g.spp, GM = g.GM;
+    const int N = g.N, c = g.c, Z = g.Z, GM = g.GM, K = g.K;
     const int lo = g.cyl_lo[col], hi = g.cyl_hi[col];
     if (lo == hi) return;
     const int gp = src_global ? view : lp;
-    const float *cp = src + ((size_t)(gp * 2 + role) * N + col) * spp;
-    for (int i = threadIdx.x; i < spp; i += blockDim.x) {
-        float v = cp[i];
-        if (apply_rescale) {
-            int s = pseg[i];
-            v = s >= 0 ? v * g.bp_rescale[s] : 0.0f;
-        }
-        colv[i] = v;
+    const float *cp = src + ((size_t)(gp * 2 + role) * N + col) * g.spp;
+    for (int i = threadIdx.x; i < g.bp_lds; i += blockDim.x) {
+        const int s0 = g.bp_src[i], s1 = g.bp_src[i + 1];
+        float v0 = 0.0f, v1 = 0.0f;
+        if (s0 >= 0) v0 = apply_rescale ? cp[s0] * g.bp_rescale[pseg[s0]] : cp[s0];
+        if (s1 >= 0) v1 = apply_rescale ? cp[s1] * g.bp_rescale[pseg[s1]] : cp[s1];
+        colpq[i] = make_float2(v0, v1 - v0);
     }
     __syncthreads();
     const int mcol = col <= c ? col : N - col;
@@ -354,8 +364,8 @@ __global__ void __launch_bounds__(256) bp_kernel(const DevGeom g, const PairBatc
     dst += (size_t)lp * N * N * g.Z4;
     const int lanei = threadIdx.x & 63, nw = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    constexpr int NS = 4;   // z slots per lane: z = lane + 64*j covers Z <= 256
-    for (int V = lo + wave; V < hi; V += nw) {
+    for (int Vv = lo + wave; Vv < hi; Vv += nw) {
+        const int V = __builtin_amdgcn_readfirstlane(Vv);
         const int X = role == 0 ? col : V, Y = role == 0 ? V : col;
         float *o = dst + ((size_t)X * N + Y) * g.Z4;
         if (masked && V != c) {
@@ -364,36 +374,55 @@ __global__ void __launch_bounds__(256) bp_kernel(const DevGeom g, const PairBatc
         }
         const int sl = V <= c ? V : N - V, ll = N - sl;
         const bool first_even = role == 0 ? (V <= c) : (V >= c);
-        float i1[NS], i2[NS];
+        // lookup entries of rows sl (lanes 0..GM-1) and ll (lanes 32..32+GM-1), with the LDS base
+        // of the segment each row reads already chosen
+        int range = 0, base = 0, frac = 0;   // frac: float bits (readlane moves 32-bit integers)
+        {
+            const bool own = lanei < GM, other = lanei >= 32 && lanei - 32 < GM;
+            if (own || other) {
+                const int4 e = g.bp_lut2[(own ? sl : ll) * GM + (lanei & 31)];
+                range = e.x;
+                base = (own == first_even) ? e.y : e.z;
+                frac = e.w;
+            }
+        }
+        float i1[4], i2[4];
 #pragma unroll
-        for (int j = 0; j < NS; j++) {
+        for (int j = 0; j < 4; j++) {
             const int z = lanei + 64 * j;
-            const float t0 = z < Z ? colv[g.seg_data[0] + z] : 0.0f;
+            const float t0 = z < Z ? colpq[z].x : 0.0f;
             i1[j] = first_even ? t0 : 0.0f;
             i2[j] = first_even ? 0.0f : t0;
         }
-        const int4 *lut1 = g.bp_lut + sl * GM, *lut2 = g.bp_lut + ll * GM;
-        for (int k = 1; k <= g.K; k++) {
-            const int4 L1 = lut1[k - 1], L2 = lut2[k - 1];       // wave-uniform (scalar) loads
-            const int2 bb = g.bp_base[k];
-            const int b1 = (first_even ? bb.x : bb.y) + L1.z, b2 = (first_even ? bb.y : bb.x) + L2.z;
-            const float f1 = __int_as_float(L1.w), f2 = __int_as_float(L2.w);
-            const int e2 = L2.x + (L1.y - L1.x);
+        for (int k = 1; k <= K; k++) {
+            const int b1 = __builtin_amdgcn_readlane(base, k - 1), b2 = __builtin_amdgcn_readlane(base, 31 + k);
+            const float f1 = __int_as_float(__builtin_amdgcn_readlane(frac, k - 1));
+            const float f2 = __int_as_float(__builtin_amdgcn_readlane(frac, 31 + k));
+            const float2 *q1 = colpq + b1 + lanei, *q2 = colpq + b2 + lanei;
+            float2 a[3], bq[3];
 #pragma unroll
-            for (int j = 0; j < NS; j++) {
-                const int z = lanei + 64 * j;
-                if (z >= L1.x && z <= L1.y) {
-                    const float p = colv[b1 + z], p1 = colv[b1 + z + 1];
-                    i1[j] = i1[j] + (p + f1 * (p1 - p));
-                }
-                if (z >= L2.x && z <= e2) {
-                    const float p = colv[b2 + z], p1 = colv[b2 + z + 1];
-                    i2[j] = i2[j] + (p + f2 * (p1 - p));
-                }
+            for (int j = 0; j < 3; j++) {
+                a[j] = q1[64 * j];
+                bq[j] = q2[64 * j];
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                i1[j] = i1[j] + (a[j].x + f1 * a[j].y);
+                i2[j] = i2[j] + (bq[j].x + f2 * bq[j].y);
+            }
+            const int r1 = __builtin_amdgcn_readlane(range, k - 1), r2 = __builtin_amdgcn_readlane(range, 31 + k);
+            const int zs1 = r1 & 0xffff, ze1 = r1 >> 16, zs2 = r2 & 0xffff;
+            if (ze1 >= 192) {
+                const float2 t = q1[192];
+                i1[3] = i1[3] + (t.x + f1 * t.y);
+            }
+            if (zs2 + (ze1 - zs1) >= 192) {
+                const float2 t = q2[192];
+                i2[3] = i2[3] + (t.x + f2 * t.y);
             }
         }
 #pragma unroll
-        for (int j = 0; j < NS; j++) {
+        for (int j = 0; j < 4; j++) {
             const int z = lanei + 64 * j;
             if (z < Z) o[z] = i1[j] + i2[j];
         }
@@ -762,8 +791,8 @@ hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *ima
 
 hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const float *src, int src_global,
                               int apply_rescale, float *bufA, float *bufB, hipStream_t s) {
-    dim3 grid(g.N - 1, 2, b.npairs), block(256);
-    size_t shm = sizeof(float) * g.spp;
+    dim3 grid(g.N - 1, 2, b.npairs), block(64 * g.bp_waves);
+    size_t shm = sizeof(float2) * g.bp_lds;
     hipLaunchKernelGGL(bp_kernel, grid, block, shm, s, g, b, src, src_global, apply_rescale, g.pseg, bufA, bufB);
     return hipGetLastError();
 }

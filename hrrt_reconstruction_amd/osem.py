@@ -172,6 +172,9 @@ class Osem3D(Layout):
     def set_launch_shape(self, tasks_per_thread: int, workgroups_per_ray: int = 0):
         _lib.call("hrrt_set_launch_shape", self.ctx, int(tasks_per_thread), int(workgroups_per_ray))
 
+    def set_backproject_shape(self, waves_per_workgroup: int):
+        _lib.call("hrrt_set_backproject_shape", self.ctx, int(waves_per_workgroup))
+
     def timing_enable(self, on: bool = True):
         _lib.call("hrrt_timing_enable", self.ctx, int(on))
 

@@ -154,6 +154,10 @@ int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream);           /
  * `fp_tasks_per_thread` (1..4) blocks of 4 planes per thread.  Results do not
  * depend on the shape. */
 int hrrt_set_launch_shape(hrrt_ctx *ctx, int fp_tasks_per_thread, int fp_workgroups_per_ray);
+/* Workgroup size of the back projector in 64-lane waves (1..16; no counterpart in
+ * the reference).  Each wave evaluates one voxel row of the column the
+ * workgroup has staged in LDS; results do not depend on the shape. */
+int hrrt_set_backproject_shape(hrrt_ctx *ctx, int waves_per_workgroup);
 int hrrt_timing_enable(hrrt_ctx *ctx, int enable);
 int hrrt_timing_read(hrrt_ctx *ctx, double *ms /* [HRRT_NKERNELS] */, int *count /* [HRRT_NKERNELS] */);
 
