@@ -1,9 +1,9 @@
 """Launch-shape autotuner for the projectors.
 
-The forward projector's workgroup owns the plane blocks of one ray column
-(one x of one view pair); the shape picks how many 4-plane blocks each
-thread accumulates in registers (tasks per thread) and how many workgroups
-share one column (each recomputing the column's rotated samples).  The back
+The forward projector's workgroup owns planes of one ray column (one x of
+one view pair); the shape picks how many planes each thread accumulates in
+registers and how many workgroups share one column (each recomputing the
+column's rotated samples).  The back
 projector's workgroup stages one sinogram column in LDS; its shape is the
 number of waves (voxel rows in flight) sharing that column.  Results are
 bit-identical for every shape; only speed differs.  The tuner times the
@@ -12,7 +12,7 @@ fastest.
 """
 from __future__ import annotations
 
-CANDIDATES = [(1, 0), (1, 3), (1, 4), (1, 6), (2, 0), (2, 2), (2, 3), (3, 0), (4, 0)]
+CANDIDATES = [(1, 0), (1, 6), (2, 0), (2, 3), (2, 4), (3, 0), (3, 3), (4, 0), (4, 2), (4, 3)]
 
 
 BP_CANDIDATES = [4, 8, 12, 16]

@@ -215,17 +215,53 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
             }
         }
     }
-    // forward-projector tasks (4-plane blocks of one segment pair), grouped by code path so
-    // that a wavefront runs one path: axial (theta = 0) blocks, oblique blocks made only of
-    // middle planes, then oblique blocks touching end planes
-    std::vector<int> tasks, t_end;
-    for (int yr = 0; yr <= G.yr_top[0]; yr += 4) tasks.push_back(yr);
-    for (int k = 1; k <= G.K; k++)
-        for (int yr = G.yr_bottom[2 * k]; yr <= G.yr_top[2 * k]; yr += 4) {
-            const bool middle = yr >= G.yr_bottom[2 * k] + ZS && yr + 3 <= G.yr_top[2 * k] - ZS;
-            (middle ? tasks : t_end).push_back((k << 16) | yr);
+    // forward-projector planes.  A plane's axial sample positions for row y are
+    // base + (y - c) * tan and base + (c - y) * tan, truncated, plus delta: end planes use
+    // their own plane and angle (flookup PJ:544-562), middle planes the first middle plane
+    // ym of the segment and its position advanced by yr - ym (PJ:1929-1944), so the device
+    // repeats the tables' float operations exactly.  Rows below ymin are skipped
+    // (PJ:1920-1921); the skip rule is checked to hold for exactly the rows below a threshold.
+    std::vector<int4> fplanes;
+    for (int yr = 0; yr < Z; yr++) fplanes.push_back(make_int4(yr << 8, 0, 0, 0));
+    for (int k = 1; k <= G.K; k++) {
+        const int th = 2 * k, yb = G.yr_bottom[th], yt = G.yr_top[th], ym = yb + ZS;
+        int ymin = 1;
+        for (int y = 1; y < c; y++)
+            if (skip[(size_t)y * GM + k - 1]) ymin = y + 1;
+        for (int y = ymin; y < c; y++)
+            if (skip[(size_t)y * GM + k - 1]) {
+                delete ctx;
+                return fail(HRRT_ERR_UNSUPPORTED, "forward-projector row skip is not a threshold (group %d)", k);
+            }
+        for (int yr = yb; yr <= yt; yr++) {
+            const bool end = yr < yb + ZS || yr > yt - ZS;
+            const int base = end ? yr : ym, delta = end ? 0 : yr - ym;
+            const float bf = (float)base;
+            const float t = G.tantheta[(size_t)(k - 1) * Z + base];
+            fplanes.push_back(make_int4(k | (yr << 8) | (delta << 16), bits(bf), bits(t), ymin));
+            // the device recomputes what the tables hold: check it on the host for every row
+            for (int y = ymin; y < c; y++) {
+                const float zz1 = bf + (y - c) * t, zz2 = bf + (c - y) * t;
+                const int z1 = (int)zz1, z2 = (int)zz2;
+                int rz, rz2;
+                float rf1, rf2;
+                const size_t li = (size_t)y * GM + k - 1;
+                if (end) {
+                    const int e = yr < yb + ZS ? yr - yb : ZS + (yr - (yt - ZS + 1));
+                    const int4 q = endp[li * 2 * ZS + e];
+                    rz = q.x; rf1 = __builtin_bit_cast(float, q.y); rz2 = q.z; rf2 = __builtin_bit_cast(float, q.w);
+                } else {
+                    const int4 m = mid[li];
+                    rz = m.x + delta; rz2 = m.y + delta;
+                    rf1 = __builtin_bit_cast(float, m.z); rf2 = __builtin_bit_cast(float, m.w);
+                }
+                if (z1 + delta != rz || z2 + delta != rz2 || bits(zz1 - z1) != bits(rf1) || bits(zz2 - z2) != bits(rf2)) {
+                    delete ctx;
+                    return fail(HRRT_ERR_UNSUPPORTED, "forward-projector positions disagree (y=%d, group=%d)", y, k);
+                }
+            }
         }
-    tasks.insert(tasks.end(), t_end.begin(), t_end.end());
+    }
     std::vector<int16_t> pseg(G.spp, -1), plane_seg(G.segmentsize, 0);
     for (int s = 0; s < G.th_pixels; s++)
         for (int i = 0; i < G.nplanes(s); i++) {
@@ -301,18 +337,17 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         (r = ctx->upload(G.yr_bottom, d.yb)) || (r = ctx->upload(G.yr_top, d.yt)) ||
         (r = ctx->upload(G.seg_data, d.seg_data)) || (r = ctx->upload(G.segment_offset, d.segment_offset)) ||
         (r = ctx->upload(file_seg, d.file_seg)) || (r = ctx->upload(G.bp_rescale, d.bp_rescale)) ||
-        (r = ctx->upload(resc, d.rescalearray)) || (r = ctx->upload(mid, d.fp_mid)) ||
-        (r = ctx->upload(skip, d.fp_skip)) || (r = ctx->upload(endp, d.fp_end)) || (r = ctx->upload(blut, d.bp_lut)) ||
-        (r = ctx->upload(rot, d.rot)) || (r = ctx->upload(tasks, d.fp_tasks)) || (r = ctx->upload(pseg, d.pseg)) ||
+        (r = ctx->upload(resc, d.rescalearray)) || (r = ctx->upload(blut, d.bp_lut)) ||
+        (r = ctx->upload(rot, d.rot)) || (r = ctx->upload(fplanes, d.fp_planes)) || (r = ctx->upload(pseg, d.pseg)) ||
         (r = ctx->upload(plane_seg, d.plane_seg)) || (r = ctx->upload(bp_src, d.bp_src)) ||
         (r = ctx->upload(blut2, d.bp_lut2))) {
         delete ctx;
         return r;
     }
-    d.ntasks = (int)tasks.size();
-    d.fp_maxt = 2;
+    d.fp_np = (int)fplanes.size();
+    d.fp_maxt = 4;
     d.fp_nsplit = 0;
-    d.bp_waves = 8;
+    d.bp_waves = 16;
     {
         std::vector<int8_t> nz((size_t)(G.views / 2) * (c + 1), 1);
         void *p = nullptr;

@@ -20,15 +20,13 @@ struct DevGeom {
     const int *file_seg;                 // [th] file plane offset (swapped segment, FIO:116) in file planes
     const float *bp_rescale;             // [th]
     const float *rescalearray;           // [GM][Z]
-    const int4 *fp_mid;                  // [N][GM]: {z, z2, bits(c1), bits(c2)}
-    const int *fp_skip;                  // [N][GM]
-    const int4 *fp_end;                  // [N][GM][2ZS]: {z, bits(f1), z2, bits(f2)}
     const int4 *bp_lut;                  // [N][GM]: {zsa, zea, off = yra - zsa, bits(frac)}
     const int4 *bp_lut2;                 // [N][GM]: {zsa | zea << 16, LDS index of plane 0 of the shifted
                                          //  segment 2k, of segment 2k-1, bits(frac)}
     const float2 *rot;                   // [views/2][N][N] rotation sample positions (rotateimage2 order)
-    const int *fp_tasks;                 // [ntasks]: (k << 16) | yr0
-    int ntasks;
+    const int4 *fp_planes;               // [fp_np] forward-projector planes, axial first, then by (k, yr):
+                                         //  {k | yr << 8 | delta << 16, bits(base plane), bits(tan), ymin}
+    int fp_np;
     const int16_t *pseg;                 // [spp] padded plane -> segment (-1 on guards)
     const int16_t *plane_seg;            // [S] internal plane -> segment
     const int *bp_src;                   // [bp_lds + 1] LDS entry -> pair-column plane (-1 = zero)

@@ -144,68 +144,60 @@ __device__ __forceinline__ void sample8_store(const DevGeom &g, const SampleRegs
 // PJ:1397-1436), optionally fused with the OP ratio (pt_update_est_w3
 // S3:2493-2505) and the back-projection rescale (PJ:1638-1649).
 //
-// One workgroup per (view pair, x in [1, N/2]).  The eight rotated z-columns
-// of each y are staged in LDS (double buffered); every thread owns up to MAXT
-// tasks of 4 consecutive planes of one segment pair and accumulates them in
-// registers over y in the reference's order.
+// One workgroup per (view pair, x in [1, N/2], share of the planes).  The
+// eight rotated z-columns of each row y are staged in LDS (double buffered,
+// the next row's samples loaded into registers while the current row is
+// projected); every thread owns up to MAXT planes (consecutive planes on
+// consecutive lanes, so LDS reads are unit-stride) and accumulates each over
+// y in the reference's order.  A plane's axial positions for row y are
+// recomputed from per-plane constants with the float operations that build
+// the reference's flookup table and middle-plane positions (PJ:544-562,
+// 1929-1944), so every lane runs one code path.
 // ---------------------------------------------------------------------------
-// Write-out of one 4-plane block of one segment for the four lanes
-// (v, x), (v, N-x), (v+90, x), (v+90, N-x) (PJ:1397-1436), optionally through
-// the OP ratio (S3:2493-2505) and the back-projection rescale (PJ:1638-1649).
+// Write-out of one plane of one segment for the four lanes (v, x), (v, N-x),
+// (v+90, x), (v+90, N-x) (PJ:1397-1436), optionally through the OP ratio
+// (S3:2493-2505) and the back-projection rescale (PJ:1638-1649).
 template <bool RATIO>
-__device__ __forceinline__ void fp_write(const DevGeom &g, const float4 (&acc)[4], int seg, int yr0, int cnt, int x,
-                                         int view, int lp, float *__restrict__ out,
-                                         const int16_t *__restrict__ prompt, const float *__restrict__ add) {
+__device__ __forceinline__ void fp_write1(const DevGeom &g, float4 acc, int seg, int yr, int x, int view, int lp,
+                                          float *__restrict__ out, const int16_t *__restrict__ prompt,
+                                          const float *__restrict__ add) {
     const int N = g.N;
-    const int pidx = g.seg_data[seg] + (yr0 - g.yb[seg]);
+    const int pidx = g.seg_data[seg] + (yr - g.yb[seg]);
     const float resc = g.bp_rescale[seg];
 #pragma unroll
     for (int l = 0; l < 4; l++) {
         if ((l & 1) && x == g.c) continue;   // at x == N/2 lanes 1/3 duplicate lanes 0/2
         const int col = (l & 1) ? N - x : x, h = l >> 1;
-        float v[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) v[r] = lane(acc[r], l);
+        float v = lane(acc, l);
         if (RATIO) {
             const size_t gi = ((size_t)(view * 2 + h) * N + col) * g.spp + pidx;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                if (r >= cnt) continue;
-                const float den = v[r] + add[gi + r];
-                const int16_t pr = prompt[gi + r];
-                float q;
-                if (pr == 0) q = 0.0f;
-                else if (den > g.eps3) q = (float)pr / den;
-                else q = 1.0f;
-                v[r] = q * resc;
-            }
+            const float den = v + add[gi];
+            const int16_t pr = prompt[gi];
+            float q;
+            if (pr == 0) q = 0.0f;
+            else if (den > g.eps3) q = (float)pr / den;
+            else q = 1.0f;
+            v = q * resc;
         }
-        float *o = out + ((size_t)(lp * 2 + h) * N + col) * g.spp + pidx;
-        if (cnt == 4) {
-            *reinterpret_cast<float4 *>(o) = make_float4(v[0], v[1], v[2], v[3]);
-        } else {
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-                if (r < cnt) o[r] = v[r];
-        }
+        out[((size_t)(lp * 2 + h) * N + col) * g.spp + pidx] = v;
     }
 }
 
 template <int MAXT, int ITEMS, bool RATIO>
-__global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatch b,
-                                                  const float4 *__restrict__ image, float *__restrict__ out,
-                                                  const int16_t *__restrict__ prompt, const float *__restrict__ add,
-                                                  int nsplit) {
+__global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBatch b,
+                                                   const float4 *__restrict__ image, float *__restrict__ out,
+                                                   const int16_t *__restrict__ prompt, const float *__restrict__ add,
+                                                   int nsplit) {
     extern __shared__ float4 lds[];
     const int PL = g.Z4 + 8;   // staged planes per column (zero beyond z_pixels)
     float4 *rimc = lds;
     float4 *stage = lds + PL;  // [2][2][PL]
-    // longest rays first (x = N/2 has the most y steps), then task split, then view pair
+    // longest rays first (x = N/2 has the most y steps), then plane share, then view pair
     const int x = g.c - (int)(blockIdx.x / nsplit), split = blockIdx.x % nsplit, lp = blockIdx.y;
-    const int t_lo = (int)(((long long)g.ntasks * split) / nsplit);
-    const int t_hi = (int)(((long long)g.ntasks * (split + 1)) / nsplit);
+    const int p_lo = (int)(((long long)g.fp_np * split) / nsplit);
+    const int p_hi = (int)(((long long)g.fp_np * (split + 1)) / nsplit);
     const int view = b.view0 + b.view_stride * lp;
-    const int N = g.N, c = g.c, GM = g.GM, ZS = g.ZS;
+    const int N = g.N, c = g.c;
     if (g.cyl_lo[x] == g.cyl_hi[x]) return;
     if (g.nzmask[view * (c + 1) + x] == 0) return;
     const float cosv = g.cos_psi[view], sinv = g.sin_psi[view];
@@ -225,25 +217,22 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
     if (ystart < c) sample8(g, image, cosv, sinv, x, ystart, stage, stage + PL);
     __syncthreads();
 
-    float4 acc1[MAXT][4], acc2[MAXT][4];
-    int tk[MAXT], ty0[MAXT], tcnt[MAXT];
+    float4 acc1[MAXT], acc2[MAXT];
+    int code[MAXT], ymin[MAXT];
+    float pbase[MAXT], ptan[MAXT];
 #pragma unroll
     for (int t = 0; t < MAXT; t++) {
-        int task = t_lo + threadIdx.x + t * blockDim.x;
-        tcnt[t] = 0; tk[t] = 0; ty0[t] = 0;
-        if (task < t_hi) {
-            int code = g.fp_tasks[task];
-            int k = code >> 16, yr0 = code & 0xffff;
-            int yt = g.yt[2 * k];
-            tk[t] = k; ty0[t] = yr0;
-            tcnt[t] = min(4, yt - yr0 + 1);
+        const int p = p_lo + threadIdx.x + t * blockDim.x;
+        code[t] = -1; ymin[t] = c; pbase[t] = 0.f; ptan[t] = 0.f;
+        int yr = 0;
+        if (p < p_hi) {
+            const int4 e = g.fp_planes[p];
+            code[t] = e.x; pbase[t] = __int_as_float(e.y); ptan[t] = __int_as_float(e.z); ymin[t] = e.w;
+            yr = (e.x >> 8) & 0xff;
         }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            float4 v = rimc[ty0[t] + r];   // planes beyond the segment are never written out
-            acc1[t][r] = v;
-            acc2[t][r] = v;
-        }
+        const float4 v = rimc[yr];   // centre row sample at plane yr (PJ:1830-1838)
+        acc1[t] = v;
+        acc2[t] = v;
     }
 
     for (int y = ystart; y < c; y++) {
@@ -251,72 +240,40 @@ __global__ void __launch_bounds__(512) fp_kernel(const DevGeom g, const PairBatc
         SampleRegs<ITEMS> nxt;
         if (y + 1 < c) sample8_load<ITEMS>(g, image, cosv, sinv, x, y + 1, nxt);
         const float4 *pa = stage + st * 2 * PL, *pb = pa + PL;
+        const float fy1 = (float)(y - c), fy2 = (float)(c - y);
 #pragma unroll
         for (int t = 0; t < MAXT; t++) {
-            const int cnt = tcnt[t], k = tk[t], yr0 = ty0[t];
-            if (cnt == 0) continue;
+            if (code[t] < 0) continue;
+            const int k = code[t] & 0xff;
             if (k == 0) {   // theta = 0 PJ:1907-1910
-                const float4 *qa = pa + yr0, *qb = pb + yr0;
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc1[t][r] = add4(acc1[t][r], add4(qa[r], qb[r]));
+                const int yr = (code[t] >> 8) & 0xff;
+                acc1[t] = add4(acc1[t], add4(pa[yr], pb[yr]));
                 continue;
             }
-            const int li = y * GM + k - 1;
-            if (g.fp_skip[li]) continue;   // PJ:1920-1921
-            const int yb = g.yb[2 * k], yt = g.yt[2 * k];
-            const int4 mid = g.fp_mid[li];
-            const float c1 = __int_as_float(mid.z), c2 = __int_as_float(mid.w);
-            if (yr0 >= yb + ZS && yr0 + 3 <= yt - ZS) {
-                // four middle planes PJ:1934-1944: constant fractions, z and z2 advance with yr;
-                // a sliding window reuses the upper samples of the previous plane
-                const int z0 = mid.x + (yr0 - (yb + ZS)), z20 = mid.y + (yr0 - (yb + ZS));
-                const float4 *qa = pa + z0, *qb = pb + z0, *ea = pa + z20, *eb = pb + z20;
-                float4 a0 = qa[0], b0 = qb[0], e0 = ea[0], d0 = eb[0];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float4 a1 = qa[r + 1], b1 = qb[r + 1], e1 = ea[r + 1], d1 = eb[r + 1];
-                    acc1[t][r] = add4(acc1[t][r], add4(mul4(add4(d1, a0), c2), mul4(add4(a1, d0), c1)));
-                    acc2[t][r] = add4(acc2[t][r], add4(mul4(add4(e1, b0), c2), mul4(add4(b1, e0), c1)));
-                    a0 = a1; b0 = b1; e0 = e1; d0 = d1;
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    if (r >= cnt) continue;
-                    const int yr = yr0 + r, i = yr - yb;
-                    int z, z2;
-                    float f1, f2;
-                    if (i < ZS || yr > yt - ZS) {   // end planes PJ:1922-1928, 1945-1950
-                        const int e = i < ZS ? i : ZS + (yr - (yt - ZS + 1));
-                        const int4 q = g.fp_end[(size_t)li * 2 * ZS + e];
-                        z = q.x; f1 = __int_as_float(q.y); z2 = q.z; f2 = __int_as_float(q.w);
-                    } else {
-                        z = mid.x + (yr - (yb + ZS)); z2 = mid.y + (yr - (yb + ZS)); f1 = c1; f2 = c2;
-                    }
-                    const float4 a0 = pa[z], a1 = pa[z + 1], b0 = pb[z], b1 = pb[z + 1];
-                    const float4 d0 = pb[z2], d1 = pb[z2 + 1], e0 = pa[z2], e1 = pa[z2 + 1];
-                    acc1[t][r] = add4(acc1[t][r], add4(mul4(add4(d1, a0), f2), mul4(add4(a1, d0), f1)));
-                    acc2[t][r] = add4(acc2[t][r], add4(mul4(add4(e1, b0), f2), mul4(add4(b1, e0), f1)));
-                }
-            }
+            if (y < ymin[t]) continue;   // PJ:1920-1921
+            const int delta = code[t] >> 16;
+            const float zz1 = pbase[t] + fy1 * ptan[t], zz2 = pbase[t] + fy2 * ptan[t];
+            const int i1 = (int)zz1, i2 = (int)zz2;
+            const float f1 = zz1 - i1, f2 = zz2 - i2;
+            const int z = i1 + delta, z2 = i2 + delta;
+            const float4 a0 = pa[z], a1 = pa[z + 1], b0 = pb[z], b1 = pb[z + 1];
+            const float4 d0 = pb[z2], d1 = pb[z2 + 1], e0 = pa[z2], e1 = pa[z2 + 1];
+            // PJ:1934-1944: (pam2[z2+1] + pam[z])*f2 + (pam[z+1] + pam2[z2])*f1 and its mirror
+            acc1[t] = add4(acc1[t], add4(mul4(add4(d1, a0), f2), mul4(add4(a1, d0), f1)));
+            acc2[t] = add4(acc2[t], add4(mul4(add4(e1, b0), f2), mul4(add4(b1, e0), f1)));
         }
         if (y + 1 < c) sample8_store<ITEMS>(g, nxt, stage + (st ^ 1) * 2 * PL, stage + (st ^ 1) * 2 * PL + PL);
         __syncthreads();
     }
 
-    // scaling PJ:1953-1975 and write-out (all loops unrolled: accumulators stay in registers)
+    // scaling PJ:1953-1975 and write-out
 #pragma unroll
     for (int t = 0; t < MAXT; t++) {
-        const int cnt = tcnt[t], k = tk[t], yr0 = ty0[t];
-        if (cnt == 0) continue;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const float sc = k == 0 ? g.y_size : g.rescalearray[(k - 1) * g.Z + min(yr0 + r, g.Z - 1)];
-            acc1[t][r] = mul4(acc1[t][r], sc);
-            acc2[t][r] = mul4(acc2[t][r], sc);
-        }
-        fp_write<RATIO>(g, acc1[t], k == 0 ? 0 : 2 * k, yr0, cnt, x, view, lp, out, prompt, add);
-        if (k != 0) fp_write<RATIO>(g, acc2[t], 2 * k - 1, yr0, cnt, x, view, lp, out, prompt, add);
+        if (code[t] < 0) continue;
+        const int k = code[t] & 0xff, yr = (code[t] >> 8) & 0xff;
+        const float sc = k == 0 ? g.y_size : g.rescalearray[(k - 1) * g.Z + yr];
+        fp_write1<RATIO>(g, mul4(acc1[t], sc), k == 0 ? 0 : 2 * k, yr, x, view, lp, out, prompt, add);
+        if (k != 0) fp_write1<RATIO>(g, mul4(acc2[t], sc), 2 * k - 1, yr, x, view, lp, out, prompt, add);
     }
 }
 
@@ -758,22 +715,17 @@ static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float 
 
 hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *image, float *out,
                           const int16_t *prompt, const float *add, int ratio_mode, hipStream_t s) {
-    // Launch shape: MAXT tasks (4-plane blocks) per thread; the tasks of one ray column
-    // are split over nsplit workgroups; each workgroup has just enough 64-lane waves
-    // (at most 8) for its share of the tasks.
+    // Launch shape: MAXT planes per thread; the planes of one ray column are split over
+    // nsplit workgroups; each workgroup has just enough 64-lane waves (at most 16) for its
+    // share of the planes and for the sampling of a row.
     int maxt = g.fp_maxt, nsplit = g.fp_nsplit;
     if (maxt < 1 || maxt > 4) maxt = 1;
-    const int need = (g.ntasks + 512 * maxt - 1) / (512 * maxt);
+    const int need = (g.fp_np + 1024 * maxt - 1) / (1024 * maxt);
     if (nsplit < need) nsplit = need;
-    const int per = (g.ntasks + nsplit - 1) / nsplit;
+    const int per = (g.fp_np + nsplit - 1) / nsplit;
     int threads = (((per + maxt - 1) / maxt) + 63) / 64 * 64;
     if (threads < 64) threads = 64;
-    const int items = (8 * g.NQ + threads - 1) / threads;
-    if (items > 2) {
-        // small workgroups: use enough threads for the sampling phase
-        threads = (8 * g.NQ + 1) / 2;
-        threads = (threads + 63) / 64 * 64;
-    }
+    if ((8 * g.NQ + threads - 1) / threads > 2) threads = ((8 * g.NQ + 1) / 2 + 63) / 64 * 64;
     const int it2 = (8 * g.NQ + threads - 1) / threads;
 #define FP_CASE(M)                                                                                    \
     case M:                                                                                           \

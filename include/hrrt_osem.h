@@ -149,9 +149,9 @@ int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream);           /
  * [1] back projection, [2] rotation + image update.  Reading resets. */
 #define HRRT_NKERNELS 3
 /* Launch shape of the forward projector (no counterpart in the reference; set
- * by the autotuner): each workgroup of 256 threads owns the plane blocks of
- * one ray column split `fp_workgroups_per_ray` ways (0 = as few as fit), with
- * `fp_tasks_per_thread` (1..4) blocks of 4 planes per thread.  Results do not
+ * by the autotuner): the planes of one ray column are split over
+ * `fp_workgroups_per_ray` workgroups (0 = as few as fit, at most 1024 threads
+ * each), with `fp_tasks_per_thread` (1..4) planes per thread.  Results do not
  * depend on the shape. */
 int hrrt_set_launch_shape(hrrt_ctx *ctx, int fp_tasks_per_thread, int fp_workgroups_per_ray);
 /* Workgroup size of the back projector in 64-lane waves (1..16; no counterpart in
