@@ -35,21 +35,25 @@ GEOM = dict(x_pixels=256, span=3, maxdel=67)
 
 
 def algorithmic_bytes(g, npairs):
-    """Bytes each kernel must move per launch (one subset), independent of the implementation."""
-    bins = npairs * 2 * g.S * g.N           # sinogram bins of the subset
-    vol = g.N * g.N * g.Z4 * 4               # one image volume in bytes
+    """Bytes each kernel must move per launch (one subset), independent of the implementation.
+
+    Only FOV bins and voxels are read or written (the reference loops over cylwiny).
+    """
+    d = g.dims
+    bins = npairs * 2 * g.S * d.fov_bins        # sinogram bins of the subset that hold a ray
+    vol = d.fov_voxels * g.Z4 * 4                # one image volume (FOV columns) in bytes
     return {
-        # image read once, int16 prompt + float additive term read, float ratio written per bin
+        # image read once; int16 prompt + float additive term read, float ratio written per bin
         "fp_ratio": vol + bins * (2 + 4 + 4),
-        # ratio read per bin, one rotated-frame volume written per view pair
-        "backproject": bins * 4 + npairs * vol,
-        # rotated-frame volumes read, image + 1/sensitivity read, image written
-        "rotate_update": npairs * vol + 3 * vol,
+        # ratio read per bin; the two rotated-frame halves of every view pair written
+        "backproject": bins * 4 + 2 * npairs * vol,
+        # both halves of every pair read; image and 1/sensitivity read, image written
+        "rotate_update": 2 * npairs * vol + 3 * vol,
     }
 
 
 def read_traffic(kernel):
-    """Measured HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    """Measured HBM bytes per launch (fetched + written) from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", "traffic_c2.txt")
     if not os.path.exists(path):
         return None

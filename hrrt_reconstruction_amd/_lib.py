@@ -38,7 +38,7 @@ class Dims(C.Structure):
         ("segmentsize", C.c_int), ("th_pixels", C.c_int), ("sviews", C.c_int), ("subsets", C.c_int),
         ("spp", C.c_int), ("x_rebin", C.c_int), ("v_rebin", C.c_int),
         ("image_floats", C.c_longlong), ("sino_bins", C.c_longlong), ("file_bins", C.c_longlong),
-        ("paircol_elems", C.c_longlong),
+        ("paircol_elems", C.c_longlong), ("fov_voxels", C.c_longlong), ("fov_bins", C.c_int),
     ]
 
 

@@ -375,6 +375,12 @@ static void fill_dims(const hrrt::Geometry &G, hrrt_dims *o) {
     o->sino_bins = (long long)G.segmentsize * G.views * G.N;
     o->file_bins = (long long)G.segmentsize * G.p.def_angles * G.p.def_elements;
     o->paircol_elems = (long long)(G.views / 2) * 2 * G.N * G.spp;
+    o->fov_voxels = 0;
+    o->fov_bins = 0;
+    for (int x = 0; x < G.N; x++) {
+        o->fov_voxels += G.cyl_hi[x] - G.cyl_lo[x];
+        if (x >= 1 && G.cyl_hi[x] > G.cyl_lo[x]) o->fov_bins++;
+    }
 }
 
 static void copy_segments(const hrrt::Geometry &G, int *yb, int *yt, int *so, int *sd) {

@@ -137,6 +137,29 @@ __device__ __forceinline__ void sample8_store(const DevGeom &g, const SampleRegs
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx k runs on XCD k mod 8),
+// each with its own L2.  This maps a dispatch index to a logical index such that every XCD
+// receives one contiguous range of logical indices, processed in order.
+__device__ __forceinline__ int xcd_contiguous(int bid, int nb) {
+    if (nb & 7) return bid;
+    return (bid & 7) * (nb >> 3) + (bid >> 3);
+}
+
+// Rotation grid: voxel column x and group of 4 rows yg.  Each XCD owns a band of row
+// groups and walks it in blocks of 16 columns, so the bilinear neighbourhoods that
+// concurrent workgroups gather stay in that XCD's L2.
+__device__ __forceinline__ void rot_block(int bid, int N, int NYG, int &x, int &yg) {
+    if ((NYG & 7) == 0 && (N & 15) == 0) {
+        const int xcd = bid & 7, r = bid >> 3;
+        const int band = NYG >> 3, blk = 16 * band;
+        x = (r / blk) * 16 + (r & 15);
+        yg = xcd * band + (r % blk) / 16;
+    } else {
+        x = bid % N;
+        yg = bid / N;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -193,7 +216,10 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
     float4 *rimc = lds;
     float4 *stage = lds + PL;  // [2][2][PL]
     // longest rays first (x = N/2 has the most y steps), then plane share, then view pair
-    const int x = g.c - (int)(blockIdx.x / nsplit), split = blockIdx.x % nsplit, lp = blockIdx.y;
+    const int per_pair = g.c * nsplit;
+    const int L = xcd_contiguous(blockIdx.x, per_pair * b.npairs);   // each XCD: consecutive rays of one pair
+    const int lp = L / per_pair, rem = L - lp * per_pair;
+    const int x = g.c - rem / nsplit, split = rem % nsplit;
     const int p_lo = (int)(((long long)g.fp_np * split) / nsplit);
     const int p_hi = (int)(((long long)g.fp_np * (split + 1)) / nsplit);
     const int view = b.view0 + b.view_stride * lp;
@@ -424,7 +450,9 @@ __device__ __forceinline__ float update1(float img, float corr, float nf, float 
 __global__ void __launch_bounds__(256) rot_update_kernel(const DevGeom g, const PairBatch b,
                                                          const float4 *__restrict__ A, const float4 *__restrict__ B,
                                                          float4 *__restrict__ image, const float4 *__restrict__ nf) {
-    const int q = threadIdx.x, x = blockIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x, yg;
+    rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
+    const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
     if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
     const float4 corr = rotate_accumulate(g, b, A, B, x, y, q);
     const size_t i = ((size_t)x * g.N + y) * g.NQ + q;
@@ -438,7 +466,9 @@ __device__ __forceinline__ float invert1(float s, float eps1) { return fabsf(s) 
 __global__ void __launch_bounds__(256) rot_sens_kernel(const DevGeom g, const PairBatch b,
                                                        const float4 *__restrict__ A, const float4 *__restrict__ B,
                                                        float4 *__restrict__ nf) {
-    const int q = threadIdx.x, x = blockIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x, yg;
+    rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
+    const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
     if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
     const float4 s = rotate_accumulate(g, b, A, B, x, y, q);
     const int z = 4 * q;
@@ -452,7 +482,9 @@ __global__ void __launch_bounds__(256) rot_sens_kernel(const DevGeom g, const Pa
 
 __global__ void __launch_bounds__(256) rot_add_kernel(const DevGeom g, int view, const float4 *__restrict__ A,
                                                       const float4 *__restrict__ B, float4 *__restrict__ ima) {
-    const int q = threadIdx.x, x = blockIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    int x, yg;
+    rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
+    const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
     if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
     PairBatch b{view, 0, 1};
     const float4 r = rotate_accumulate(g, b, A, B, x, y, q);
@@ -702,7 +734,7 @@ template <int MAXT, int ITEMS>
 static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float *image, float *out,
                               const int16_t *prompt, const float *add, int ratio, int nsplit, int threads,
                               hipStream_t s) {
-    dim3 grid(g.c * nsplit, b.npairs), block(threads);
+    dim3 grid(g.c * nsplit * b.npairs), block(threads);
     size_t shm = sizeof(float4) * 5 * (g.Z4 + 8);
     if (ratio)
         hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, true>), grid, block, shm, s, g, b,
@@ -749,7 +781,7 @@ hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const float 
     return hipGetLastError();
 }
 
-static dim3 rot_grid(const DevGeom &g) { return dim3(g.N, (g.N + 3) / 4); }
+static dim3 rot_grid(const DevGeom &g) { return dim3(g.N * ((g.N + 3) / 4)); }
 
 hipError_t launch_rotate_update(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
                                 float *image, const float *normfac, hipStream_t s) {

@@ -77,6 +77,8 @@ typedef struct {
     long long sino_bins;          /* S*views*N (internal sinogram)        */
     long long file_bins;          /* S*def_angles*def_elements (file)     */
     long long paircol_elems;      /* (views/2)*2*N*spp                    */
+    long long fov_voxels;         /* (x, y) image columns inside the FOV (cylwiny, PJ:339-356) */
+    int fov_bins;                 /* radial bins 1..N-1 whose image row holds FOV pixels */
 } hrrt_dims;
 
 /* Create a context: GetScannerParameter (S3:3452) + dependencies (PJ:199).

@@ -78,6 +78,9 @@ def test_query_geometry_matches_oracle(name, oracle_lib):
     assert L.dims.sino_bins == L.S * L.views * L.N
     assert L.dims.file_bins == L.S * o.params.def_angles * o.params.def_elements
     assert L.dims.paircol_elems == L.views // 2 * 2 * L.N * L.spp
+    cyl = o.cylwiny
+    assert L.dims.fov_voxels == int((cyl[:, 1] - cyl[:, 0]).sum())
+    assert L.dims.fov_bins == int(sum(1 for x in range(1, L.N) if cyl[x, 1] > cyl[x, 0]))
 
 
 def test_paircol_layout_round_trip():
