@@ -16,6 +16,7 @@ CalculateOsem3d inner body.  Steps cycle through the reference's subset order.
 from __future__ import annotations
 
 import argparse
+import math
 import json
 import os
 import sys
@@ -126,6 +127,107 @@ def cpu_baseline(g, torch, prompt, add, normfac, image, nsub, threads):
     return nsub / dt, dt
 
 
+DYN_GEOM = dict(x_pixels=128, span=9, maxdel=67)
+
+
+def batched_frames(torch, nframes, iterations, seed):
+    """Dynamic study in batched mode: every frame's subsets run in one launch per kernel, each subset
+    of an iteration replayed from a HIP graph.  FP32 form and FP16-input precision form."""
+    from hrrt_reconstruction_amd import phantom as ph
+    from hrrt_reconstruction_amd.osem import Osem3D
+    g = Osem3D("cuda:0", **DYN_GEOM)
+    p = g.params
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    cg = torch.Generator()
+    cg.manual_seed(seed)
+    dur, mids, counts, rates = ph.dynamic_frames(nframes, 0.5e6, 200e6, cg)
+    pc = (g.views // 2, 2, g.N, g.spp)
+    prompts = torch.zeros((nframes,) + pc, dtype=torch.int16, device="cuda")
+    adds = torch.zeros((nframes,) + pc, dtype=torch.float32, device="cuda")
+    est = g.estimate_zeros()
+    norm = atten = invnorm = None
+    for f in range(nframes):
+        act = [1.0 + 3.0 * math.exp(-r * mids[f]) for r in rates]
+        pg = torch.Generator()
+        pg.manual_seed(seed + 1)                 # same sphere positions in every frame
+        img, _ = ph.make_phantom(g.N, g.Z4, g.Z, p.bin_size * g.dims.x_rebin, p.plane_sep, 6, 200, pg, "cuda",
+                                 sphere_activity=act)
+        proj = torch.zeros((g.S, g.views, g.N), dtype=torch.float32, device="cuda")
+        for v in range(g.views // 2):
+            g.forward_proj3d(img, v, prj=est)
+            proj[:, v, :] = est[: g.S]
+            proj[:, v + g.views // 2, :] = est[g.S:]
+        # file-layout sinograms: rebin the projections back to 256 x 288 by repetition
+        rb = g.dims.x_rebin
+        proj_file = proj.repeat_interleave(rb, 1).repeat_interleave(rb, 2) / (rb * rb)
+        st = ph.make_study(proj_file, p.rings, p.span, p.maxdel, p.def_elements, p.def_angles, p.bin_size,
+                           p.plane_sep, p.crystal_radius, counts[f], gen, "cuda", with_scatter=False,
+                           norm=norm, atten=atten)
+        norm, atten = st["norm"], st["atten"]
+        pr, ad, inv = g.stage(st["prompt"], st["delayed"], st["norm"], None, st["atten"])
+        prompts[f].copy_(pr)
+        adds[f].copy_(ad)
+        if invnorm is None:
+            invnorm = inv
+        del proj, proj_file, st, pr, ad, inv
+    torch.cuda.synchronize()
+    log(f"dynamic study: {nframes} frames generated and staged")
+    nf32 = g.calculate_normfac(invnorm)
+    inv16 = g.to_half(invnorm)
+    nf16 = g.calculate_normfac_mixed(inv16)
+    adds16 = g.to_half(adds)
+    stream = torch.cuda.Stream()
+    res = {}
+    for form in ("fp32", "fp16"):
+        imgs = torch.stack([g.init_image() for _ in range(nframes)])
+        ll = torch.zeros((g.subsets, nframes), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for _ in range(iterations):
+                if form == "fp32":
+                    g.osem_iteration_frames(imgs, nf32, prompts, adds)
+                else:
+                    g.osem_iteration_frames_mixed(imgs, nf16, prompts, adds16, invnorm16=inv16, loglik=ll)
+        stream.synchronize()
+        dt = time.perf_counter() - t0
+        res[form] = (imgs, dt, ll)
+        log(f"batched {form}: {nframes} frames x {g.subsets} subsets x {iterations} iterations in {dt:.3f} s "
+            f"(graphs captured on first use)")
+    # time again with the graphs already instantiated
+    out = {}
+    for form in ("fp32", "fp16"):
+        imgs = torch.stack([g.init_image() for _ in range(nframes)])
+        ll = res[form][2]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            for _ in range(iterations):
+                if form == "fp32":
+                    g.osem_iteration_frames(imgs, nf32, prompts, adds)
+                else:
+                    g.osem_iteration_frames_mixed(imgs, nf16, prompts, adds16, invnorm16=inv16, loglik=ll)
+        stream.synchronize()
+        out[form] = time.perf_counter() - t0
+        res[form] = (imgs, out[form], ll)
+    a, b = res["fp32"][0], res["fp16"][0]
+    rel = [float(torch.linalg.vector_norm((b[f] - a[f]).double()) / torch.linalg.vector_norm(a[f].double()))
+           for f in range(nframes)]
+    return {
+        "workload": f"dynamic study, HRRT span 9, 128x128x207 image, {nframes} frames "
+                    f"({counts[0] / 1e6:.2f}M..: log-uniform 0.5M-200M prompts, spheres washing out, randoms 10%), "
+                    f"{g.subsets} subsets x {iterations} iterations, one launch per kernel for all frames, "
+                    "subsets replayed from HIP graphs",
+        "frames": nframes, "iterations": iterations,
+        "fp32_frames_per_s": round(nframes / out["fp32"], 3),
+        "fp16_frames_per_s": round(nframes / out["fp16"], 3),
+        "fp16_vs_fp32": round(out["fp32"] / out["fp16"], 3),
+        "fp16_image_rel_l2_vs_fp32_max": float(f"{max(rel):.3e}"),
+        "fp16_loglik_last_subset": [float(x) for x in res["fp16"][2][-1].cpu().numpy()[:3]],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,10 +237,12 @@ def main():
     ap.add_argument("--cpu-subsets", type=int, default=4, help="timed subsets of the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-tune", action="store_true", help="skip the forward-projector launch-shape autotuner")
+    ap.add_argument("--frames", type=int, default=24, help="frames of the batched dynamic study (0 = skip)")
+    ap.add_argument("--frame-iterations", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(frames and view pairs would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(frames and view pairs would shard across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -213,6 +317,10 @@ def main():
                "sample": f"{args.cpu_subsets} OSEM subset updates of the same workload after one warm-up "
                          f"subset, reference thread schedule T={args.cpu_threads} ({dt:.1f} s)"}
 
+    batched = None
+    if args.frames > 0:
+        batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
+
     out = {
         "metric": "OSEM subset-iters/s span3 256²×207; batched frames/s; % HBM roofline",
         "value": round(value, 3),
@@ -237,6 +345,7 @@ def main():
         "kernels": {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 4), "launches": v[1],
                         "alg_bytes": alg[k]} for k, v in kt.items()},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -59,6 +59,14 @@ SIGNATURES = {
     "hrrt_init_image": (_I, [_VP, _VP, _VP]),
     "hrrt_osem_subset": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_end_iteration": (_I, [_VP, _VP, _VP]),
+    "hrrt_osem_subset_frames": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, _VP]),
+    "hrrt_osem_iteration_frames": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
+    "hrrt_to_half": (_I, [_VP, _VP, C.c_longlong, _VP]),
+    "hrrt_calculate_normfac_mixed": (_I, [_VP, _VP, _VP, _VP]),
+    "hrrt_osem_subset_mixed": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
+    "hrrt_osem_subset_frames_mixed": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
+    "hrrt_osem_iteration_frames_mixed": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "hrrt_count_moments": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "hrrt_timing_enable": (_I, [_VP, _I]),
     "hrrt_set_launch_shape": (_I, [_VP, _I, _I]),
     "hrrt_set_backproject_shape": (_I, [_VP, _I]),

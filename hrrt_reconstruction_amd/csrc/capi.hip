@@ -108,10 +108,62 @@ struct hrrt_ctx {
         if ((r = zalloc((void **)&bufB, sizeof(float) * vol_floats() * (npairs() > 0 ? npairs() : 1)))) return r;
         return HRRT_OK;
     }
+    // batched dynamic frames: per-frame scratch (zeroed once: voxels outside the FOV are
+    // never written and must read as zero) and one instantiated graph per subset
+    float *f_ratio = nullptr, *fA = nullptr, *fB = nullptr;
+    int f_cap = 0;
+    struct GraphKey {
+        int nframes = 0;
+        const void *img = nullptr, *nf = nullptr, *prompt = nullptr, *add = nullptr;
+        void *stream = nullptr;
+        bool mixed = false;
+        const void *inv = nullptr, *ll = nullptr;
+        bool operator==(const GraphKey &o) const {
+            return nframes == o.nframes && img == o.img && nf == o.nf && prompt == o.prompt && add == o.add &&
+                   stream == o.stream && mixed == o.mixed && inv == o.inv && ll == o.ll;
+        }
+    } gkey;
+    double *ll_part = nullptr;           // FP64 partial sums (log-likelihood, count moments)
+    size_t ll_cap = 0;
+    int ensure_ll_part(size_t n) {
+        if (n <= ll_cap) return HRRT_OK;
+        if (ll_part) (void)hipFree(ll_part);
+        ll_part = nullptr;
+        ll_cap = 0;
+        drop_graphs();
+        int r;
+        if ((r = zalloc((void **)&ll_part, sizeof(double) * n))) return r;
+        ll_cap = n;
+        return HRRT_OK;
+    }
+    std::vector<hipGraphExec_t> gexec;   // [subsets]
+    int ensure_frame_scratch(int nframes) {
+        if (nframes <= f_cap) return HRRT_OK;
+        void *s[] = {f_ratio, fA, fB};
+        for (void *p : s)
+            if (p) (void)hipFree(p);
+        f_ratio = fA = fB = nullptr;
+        f_cap = 0;
+        int r;
+        const size_t nr = col_floats() * npairs() * nframes, nb = vol_floats() * npairs() * nframes;
+        if ((r = zalloc((void **)&f_ratio, sizeof(float) * nr))) return r;
+        if ((r = zalloc((void **)&fA, sizeof(float) * nb))) return r;
+        if ((r = zalloc((void **)&fB, sizeof(float) * nb))) return r;
+        f_cap = nframes;
+        return HRRT_OK;
+    }
+    void drop_graphs() {
+        for (hipGraphExec_t e : gexec)
+            if (e) (void)hipGraphExecDestroy(e);
+        gexec.clear();
+        gkey = GraphKey{};
+    }
     ~hrrt_ctx() {
+        drop_graphs();
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
-        void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, (void *)dg.nzmask};
+        void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, (void *)dg.nzmask, f_ratio, fA, fB,
+                     ll_part};
         for (void *p : s)
             if (p) (void)hipFree(p);
     }
@@ -462,9 +514,9 @@ extern "C" int hrrt_calculate_normfac(hrrt_ctx *ctx, const float *d_invnorm, flo
     const hrrt::Geometry &G = ctx->geo;
     for (int s = 0; s < G.p.subsets; s++) {
         hrrt::PairBatch b{s, G.p.subsets, ctx->npairs()};
-        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, d_invnorm, 1, 1, ctx->bufA, ctx->bufB, S(stream)));
+        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, d_invnorm, 0, 1, 1, 0, ctx->bufA, ctx->bufB, S(stream)));
         HIP_TRY(hrrt::launch_rotate_sens(ctx->dg, b, ctx->bufA, ctx->bufB, d_normfac + (size_t)s * ctx->vol_floats(),
-                                         S(stream)));
+                                         0, S(stream)));
     }
     return HRRT_OK;
 }
@@ -490,12 +542,208 @@ extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_no
     if ((r = ctx->ensure_subset_scratch())) return r;
     hrrt::PairBatch b{isubset, G.p.subsets, ctx->npairs()};
     ctx->mark(0, S(stream));
-    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, d_prompt, d_add, 1, S(stream)));
+    hrrt::FpIO io;
+    io.prompt = d_prompt;
+    io.add = d_add;
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, io, hrrt::FP_RATIO, S(stream)));
     ctx->mark(1, S(stream));
-    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->sub_ratio, 0, 0, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
     ctx->mark(2, S(stream));
-    HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, S(stream)));
+    HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, 0, S(stream)));
     ctx->mark(-1, S(stream));
+    return HRRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched dynamic frames: every frame of a study runs the same subset in one
+// launch per kernel (the frame is a grid dimension), frames at fixed strides.
+// ---------------------------------------------------------------------------
+static hrrt::PairBatch frame_batch(const hrrt_ctx *ctx, int isubset, int nframes) {
+    const hrrt::Geometry &G = ctx->geo;
+    hrrt::PairBatch b{isubset, G.p.subsets, ctx->npairs()};
+    b.nframes = nframes;
+    b.img_stride = (long long)ctx->vol_floats();
+    b.sino_stride = (long long)(G.views / 2) * ctx->col_floats();
+    b.ratio_stride = (long long)ctx->col_floats() * ctx->npairs();
+    b.buf_stride = (long long)ctx->vol_floats() * ctx->npairs();
+    return b;
+}
+
+// precision form of a launch: plain FP32 (bit-exact) or FP16 per-bin inputs with Kahan accumulation
+struct Precision {
+    bool mixed = false;
+    const uint16_t *invnorm16 = nullptr;   // for the log-likelihood
+    double *loglik = nullptr;              // [nframes] (NULL: not computed)
+};
+
+static int frames_subset_launch(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac_subset,
+                                const int16_t *d_prompts, const void *d_adds, int isubset, hipStream_t s,
+                                const Precision &pr = Precision()) {
+    const hrrt::PairBatch b = frame_batch(ctx, isubset, nframes);
+    hrrt::FpIO io;
+    io.prompt = d_prompts;
+    io.add = d_adds;
+    if (pr.mixed && pr.loglik) {
+        io.invnorm = pr.invnorm16;
+        io.ll_part = ctx->ll_part;
+    }
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_images, ctx->f_ratio, io, pr.mixed ? hrrt::FP_RATIO16 : hrrt::FP_RATIO, s));
+    if (pr.mixed && pr.loglik) {
+        int maxt, nsplit, threads, items;
+        hrrt::forward_shape(ctx->dg, maxt, nsplit, threads, items);
+        HIP_TRY(hrrt::launch_ll_reduce(ctx->ll_part, ctx->dg.c * nsplit * b.npairs, nframes, pr.loglik, s));
+    }
+    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->f_ratio, 0, 0, 0, pr.mixed, ctx->fA, ctx->fB, s));
+    HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->fA, ctx->fB, d_images, d_normfac_subset, pr.mixed, s));
+    return HRRT_OK;
+}
+
+static int frames_check(hrrt_ctx *ctx, int nframes, const void *a, const void *b, const void *c, const void *d) {
+    if (!ctx || !a || !b || !c || !d) return fail(HRRT_ERR_ARG, "null argument");
+    if (nframes < 1) return fail(HRRT_ERR_ARG, "nframes must be >= 1");
+    const long long blocks = (long long)ctx->geo.N / 2 * 64 * ctx->npairs() * nframes;   // forward grid bound
+    if (blocks > 0x7fffffffLL || (long long)ctx->npairs() * nframes > 65535)
+        return fail(HRRT_ERR_ARG, "too many frames for one launch (%d)", nframes);
+    int r;
+    if ((r = ctx->ensure_frame_scratch(nframes))) return r;
+    int maxt, nsplit, threads, items;
+    hrrt::forward_shape(ctx->dg, maxt, nsplit, threads, items);
+    return ctx->ensure_ll_part((size_t)ctx->dg.c * nsplit * ctx->npairs() * nframes);
+}
+
+extern "C" int hrrt_osem_subset_frames(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac_subset,
+                                       const int16_t *d_prompts, const float *d_adds, int isubset, void *stream) {
+    int r;
+    if ((r = frames_check(ctx, nframes, d_images, d_normfac_subset, d_prompts, d_adds))) return r;
+    if (isubset < 0 || isubset >= ctx->geo.p.subsets) return fail(HRRT_ERR_ARG, "subset %d out of range", isubset);
+    return frames_subset_launch(ctx, nframes, d_images, d_normfac_subset, d_prompts, d_adds, isubset, S(stream));
+}
+
+static int iteration_frames(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac,
+                            const int16_t *d_prompts, const void *d_adds, void *stream, bool mixed,
+                            const uint16_t *d_invnorm16, double *d_loglik) {
+    int r;
+    if ((r = frames_check(ctx, nframes, d_images, d_normfac, d_prompts, d_adds))) return r;
+    if (!stream) return fail(HRRT_ERR_ARG, "graph replay needs a non-default stream");
+    if (d_loglik && !d_invnorm16) return fail(HRRT_ERR_ARG, "the log-likelihood needs the 1/(N*A) sinogram");
+    const hrrt::Geometry &G = ctx->geo;
+    hipStream_t s = S(stream);
+    hrrt_ctx::GraphKey key;
+    key.nframes = nframes; key.img = d_images; key.nf = d_normfac; key.prompt = d_prompts; key.add = d_adds;
+    key.stream = stream; key.mixed = mixed; key.inv = d_invnorm16; key.ll = d_loglik;
+    if (!(key == ctx->gkey) || (int)ctx->gexec.size() != G.p.subsets) {
+        ctx->drop_graphs();
+        for (int i = 0; i < G.p.subsets; i++) {
+            const int isub = G.order[i];
+            Precision pr;
+            pr.mixed = mixed;
+            pr.invnorm16 = d_invnorm16;
+            pr.loglik = d_loglik ? d_loglik + (size_t)isub * nframes : nullptr;
+            hipGraph_t graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = frames_subset_launch(ctx, nframes, d_images, d_normfac + (size_t)isub * ctx->vol_floats(),
+                                          d_prompts, d_adds, isub, s, pr);
+            hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                ctx->drop_graphs();
+                return rc;
+            }
+            if (e != hipSuccess) {
+                ctx->drop_graphs();
+                return fail(HRRT_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            }
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                ctx->drop_graphs();
+                return fail(HRRT_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            }
+            ctx->gexec.push_back(ex);
+        }
+        ctx->gkey = key;
+    }
+    for (int i = 0; i < G.p.subsets; i++) HIP_TRY(hipGraphLaunch(ctx->gexec[i], s));
+    for (int f = 0; f < nframes; f++)
+        HIP_TRY(hrrt::launch_end_iteration(ctx->dg, d_images + (size_t)f * ctx->vol_floats(), G.z_min, G.z_max, s));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_osem_iteration_frames(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac,
+                                          const int16_t *d_prompts, const float *d_adds, void *stream) {
+    return iteration_frames(ctx, nframes, d_images, d_normfac, d_prompts, d_adds, stream, false, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Precision form: FP16 per-bin inputs, Kahan accumulation, FP64 log-likelihood
+// and count normalisation.
+// ---------------------------------------------------------------------------
+extern "C" int hrrt_to_half(const float *d_in, uint16_t *d_out, long long n, void *stream) {
+    if (!d_in || !d_out || n < 0) return fail(HRRT_ERR_ARG, "invalid argument");
+    HIP_TRY(hrrt::launch_to_half(d_in, d_out, (size_t)n, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_calculate_normfac_mixed(hrrt_ctx *ctx, const uint16_t *d_invnorm16, float *d_normfac,
+                                            void *stream) {
+    if (!ctx || !d_invnorm16 || !d_normfac) return fail(HRRT_ERR_ARG, "null argument");
+    int r;
+    if ((r = ctx->ensure_subset_scratch())) return r;
+    const hrrt::Geometry &G = ctx->geo;
+    for (int s = 0; s < G.p.subsets; s++) {
+        hrrt::PairBatch b{s, G.p.subsets, ctx->npairs()};
+        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, d_invnorm16, 1, 1, 1, 1, ctx->bufA, ctx->bufB, S(stream)));
+        HIP_TRY(hrrt::launch_rotate_sens(ctx->dg, b, ctx->bufA, ctx->bufB, d_normfac + (size_t)s * ctx->vol_floats(),
+                                         1, S(stream)));
+    }
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_osem_subset_frames_mixed(hrrt_ctx *ctx, int nframes, float *d_images,
+                                             const float *d_normfac_subset, const int16_t *d_prompts,
+                                             const uint16_t *d_adds16, const uint16_t *d_invnorm16, int isubset,
+                                             double *d_loglik, void *stream) {
+    int r;
+    if ((r = frames_check(ctx, nframes, d_images, d_normfac_subset, d_prompts, d_adds16))) return r;
+    if (isubset < 0 || isubset >= ctx->geo.p.subsets) return fail(HRRT_ERR_ARG, "subset %d out of range", isubset);
+    if (d_loglik && !d_invnorm16) return fail(HRRT_ERR_ARG, "the log-likelihood needs the 1/(N*A) sinogram");
+    Precision pr;
+    pr.mixed = true;
+    pr.invnorm16 = d_invnorm16;
+    pr.loglik = d_loglik;
+    return frames_subset_launch(ctx, nframes, d_images, d_normfac_subset, d_prompts, d_adds16, isubset, S(stream), pr);
+}
+
+extern "C" int hrrt_osem_subset_mixed(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subset,
+                                      const int16_t *d_prompt, const uint16_t *d_add16, const uint16_t *d_invnorm16,
+                                      int isubset, double *d_loglik, void *stream) {
+    return hrrt_osem_subset_frames_mixed(ctx, 1, d_image, d_normfac_subset, d_prompt, d_add16, d_invnorm16, isubset,
+                                         d_loglik, stream);
+}
+
+extern "C" int hrrt_osem_iteration_frames_mixed(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac,
+                                                const int16_t *d_prompts, const uint16_t *d_adds16,
+                                                const uint16_t *d_invnorm16, double *d_loglik, void *stream) {
+    return iteration_frames(ctx, nframes, d_images, d_normfac, d_prompts, d_adds16, stream, true, d_invnorm16,
+                            d_loglik);
+}
+
+extern "C" int hrrt_count_moments(hrrt_ctx *ctx, const float *d_image, const int16_t *d_prompt, const float *d_add,
+                                  const float *d_invnorm, double *d_out, void *stream) {
+    if (!ctx || !d_image || !d_prompt || !d_add || !d_invnorm || !d_out) return fail(HRRT_ERR_ARG, "null argument");
+    int r;
+    if ((r = ctx->ensure_subset_scratch())) return r;
+    const hrrt::Geometry &G = ctx->geo;
+    if ((r = ctx->ensure_ll_part((size_t)3 * G.N * 2 * ctx->npairs()))) return r;
+    HIP_TRY(hipMemsetAsync(d_out, 0, 3 * sizeof(double), S(stream)));
+    hrrt::FpIO io;
+    for (int s = 0; s < G.p.subsets; s++) {
+        hrrt::PairBatch b{s, G.p.subsets, ctx->npairs()};
+        HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, io, hrrt::FP_PLAIN, S(stream)));
+        HIP_TRY(hrrt::launch_count_moments(ctx->dg, d_prompt, d_add, d_invnorm, ctx->sub_ratio, s, G.p.subsets,
+                                           ctx->npairs(), ctx->ll_part, d_out, S(stream)));
+    }
     return HRRT_OK;
 }
 
@@ -505,6 +753,7 @@ extern "C" int hrrt_set_launch_shape(hrrt_ctx *ctx, int fp_tasks_per_thread, int
         return fail(HRRT_ERR_ARG, "launch shape out of range");
     ctx->dg.fp_maxt = fp_tasks_per_thread;
     ctx->dg.fp_nsplit = fp_workgroups_per_ray;
+    ctx->drop_graphs();
     return HRRT_OK;
 }
 
@@ -512,6 +761,7 @@ extern "C" int hrrt_set_backproject_shape(hrrt_ctx *ctx, int waves_per_workgroup
     if (!ctx) return fail(HRRT_ERR_ARG, "null context");
     if (waves_per_workgroup < 1 || waves_per_workgroup > 16) return fail(HRRT_ERR_ARG, "launch shape out of range");
     ctx->dg.bp_waves = waves_per_workgroup;
+    ctx->drop_graphs();
     return HRRT_OK;
 }
 
@@ -544,7 +794,7 @@ extern "C" int hrrt_forward_proj3d(hrrt_ctx *ctx, const float *d_image, float *d
     int r;
     if ((r = ctx->ensure_pair_scratch())) return r;
     hrrt::PairBatch b{view, 0, 1};
-    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->pc1, nullptr, nullptr, 0, S(stream)));
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->pc1, hrrt::FpIO(), hrrt::FP_PLAIN, S(stream)));
     HIP_TRY(hrrt::launch_paircol_to_estimate(ctx->dg, view, ctx->pc1, d_prj, S(stream)));
     return HRRT_OK;
 }
@@ -556,7 +806,7 @@ extern "C" int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int v
     if ((r = ctx->ensure_pair_scratch())) return r;
     hrrt::PairBatch b{view, 0, 1};
     HIP_TRY(hrrt::launch_estimate_to_paircol(ctx->dg, d_prj, ctx->pc1, S(stream)));
-    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->pc1, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->pc1, 0, 0, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
     HIP_TRY(hrrt::launch_rotate_add(ctx->dg, view, ctx->bufA, ctx->bufB, d_ima, S(stream)));
     return HRRT_OK;
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 
 #include <cstdint>
 
@@ -36,19 +37,48 @@ struct DevGeom {
     int bp_waves;                        // back-projector waves per workgroup
 };
 
-// One batch of view pairs processed together (a subset, or a single pair).
+// One batch of view pairs processed together (a subset, or a single pair), for one
+// frame or for nframes frames of a dynamic study laid out at fixed strides.
 struct PairBatch {
     int view0, view_stride, npairs;      // view of local pair lp = view0 + view_stride*lp
+    int nframes = 1;
+    long long img_stride = 0;            // floats between frames' images
+    long long sino_stride = 0;           // elements between frames' pair-column prompts / additive terms
+    long long ratio_stride = 0;          // floats between frames' ratio buffers
+    long long buf_stride = 0;            // floats between frames' rotated-frame buffers
+};
+
+// Per-bin inputs of the forward projector's epilogue.
+//   FP_PLAIN  : projection only (out = scaled estimate)
+//   FP_RATIO  : OP ratio with float additive term (bit-exact path)
+//   FP_RATIO16: OP ratio with FP16 additive term; with ll_part set, also the FP64
+//               Poisson log-likelihood of the subset's bins (needs the FP16 1/(N*A))
+enum FpMode { FP_PLAIN = 0, FP_RATIO = 1, FP_RATIO16 = 2 };
+struct FpIO {
+    const int16_t *prompt = nullptr;
+    const void *add = nullptr;           // float (FP_RATIO) or FP16 bits (FP_RATIO16)
+    const uint16_t *invnorm = nullptr;   // FP16 1/(N*A) (FP_RATIO16 log-likelihood)
+    double *ll_part = nullptr;           // one partial sum per workgroup (FP_RATIO16)
 };
 
 hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *image, float *out_paircol,
-                          const int16_t *prompt, const float *add, int ratio_mode, hipStream_t s);
-hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const float *src_paircol, int src_global,
-                              int apply_rescale, float *bufA, float *bufB, hipStream_t s);
+                          const FpIO &io, int mode, hipStream_t s);
+// launch shape the forward projector uses (workgroups per ray = nsplit, planes per thread = maxt)
+void forward_shape(const DevGeom &g, int &maxt, int &nsplit, int &threads, int &items);
+// src: float pair-column, or FP16 bits when src_half; kahan: compensated accumulation over groups
+hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const void *src_paircol, int src_half,
+                              int src_global, int apply_rescale, int kahan, float *bufA, float *bufB, hipStream_t s);
 hipError_t launch_rotate_update(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
-                                float *image, const float *normfac, hipStream_t s);
+                                float *image, const float *normfac, int kahan, hipStream_t s);
 hipError_t launch_rotate_sens(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
-                              float *normfac_out, hipStream_t s);
+                              float *normfac_out, int kahan, hipStream_t s);
+// per-frame sums of the forward projector's log-likelihood partials (fixed order, deterministic)
+hipError_t launch_ll_reduce(const double *part, int per_frame, int nframes, double *out, hipStream_t s);
+hipError_t launch_to_half(const float *in, uint16_t *out, size_t n, hipStream_t s);
+// FP64 sums over a pair-column sinogram: [0] sum prompt, [1] sum w*add, [2] sum w*est, w = 1/(N*A)
+hipError_t launch_count_moments(const DevGeom &g, const int16_t *prompt, const float *add, const float *invnorm,
+                                const float *est, int view0, int view_stride, int npairs, double *part,
+                                double *out, hipStream_t s);
 hipError_t launch_rotate_add(const DevGeom &g, int view, const float *bufA, const float *bufB, float *ima,
                              hipStream_t s);
 hipError_t launch_paircol_to_estimate(const DevGeom &g, int view, const float *paircol, float *prj, hipStream_t s);

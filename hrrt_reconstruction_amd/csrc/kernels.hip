@@ -24,6 +24,16 @@ __device__ __forceinline__ float4 sub4(float4 a, float4 b) {
 __device__ __forceinline__ float4 mul4(float4 a, float s) {
     return make_float4(a.x * s, a.y * s, a.z * s, a.w * s);
 }
+__device__ __forceinline__ float half_at(const uint16_t *p, size_t i) { return __half2float(__ushort_as_half(p[i])); }
+
+// Compensated (Kahan) summation: s accumulates, c carries the low-order bits lost so far.
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
 __device__ __forceinline__ float lane(const float4 &v, int l) {
     return l == 0 ? v.x : (l == 1 ? v.y : (l == 2 ? v.z : v.w));
 }
@@ -180,10 +190,9 @@ __device__ __forceinline__ void rot_block(int bid, int N, int NYG, int &x, int &
 // Write-out of one plane of one segment for the four lanes (v, x), (v, N-x),
 // (v+90, x), (v+90, N-x) (PJ:1397-1436), optionally through the OP ratio
 // (S3:2493-2505) and the back-projection rescale (PJ:1638-1649).
-template <bool RATIO>
+template <int MODE>
 __device__ __forceinline__ void fp_write1(const DevGeom &g, float4 acc, int seg, int yr, int x, int view, int lp,
-                                          float *__restrict__ out, const int16_t *__restrict__ prompt,
-                                          const float *__restrict__ add) {
+                                          float *__restrict__ out, const FpIO &io, double &ll) {
     const int N = g.N;
     const int pidx = g.seg_data[seg] + (yr - g.yb[seg]);
     const float resc = g.bp_rescale[seg];
@@ -192,40 +201,61 @@ __device__ __forceinline__ void fp_write1(const DevGeom &g, float4 acc, int seg,
         if ((l & 1) && x == g.c) continue;   // at x == N/2 lanes 1/3 duplicate lanes 0/2
         const int col = (l & 1) ? N - x : x, h = l >> 1;
         float v = lane(acc, l);
-        if (RATIO) {
+        if (MODE != FP_PLAIN) {
             const size_t gi = ((size_t)(view * 2 + h) * N + col) * g.spp + pidx;
-            const float den = v + add[gi];
-            const int16_t pr = prompt[gi];
+            const float a = MODE == FP_RATIO ? static_cast<const float *>(io.add)[gi]
+                                             : half_at(static_cast<const uint16_t *>(io.add), gi);
+            const float den = v + a;
+            const int16_t pr = io.prompt[gi];
             float q;
             if (pr == 0) q = 0.0f;
             else if (den > g.eps3) q = (float)pr / den;
             else q = 1.0f;
+            if (MODE == FP_RATIO16 && io.ll_part) {
+                // Poisson log-likelihood of the bin (without log y!) at the mean (est + add) / (N*A)
+                const float w = half_at(io.invnorm, gi);
+                if (w > 0.0f && den > g.eps3) {
+                    const double mu = (double)den * (double)w;
+                    ll += (pr > 0 ? (double)pr * log(mu) : 0.0) - mu;
+                }
+            }
             v = q * resc;
         }
         out[((size_t)(lp * 2 + h) * N + col) * g.spp + pidx] = v;
     }
 }
 
-template <int MAXT, int ITEMS, bool RATIO>
+template <int MAXT, int ITEMS, int MODE>
 __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBatch b,
                                                    const float4 *__restrict__ image, float *__restrict__ out,
-                                                   const int16_t *__restrict__ prompt, const float *__restrict__ add,
-                                                   int nsplit) {
+                                                   FpIO io, int nsplit) {
     extern __shared__ float4 lds[];
     const int PL = g.Z4 + 8;   // staged planes per column (zero beyond z_pixels)
     float4 *rimc = lds;
     float4 *stage = lds + PL;  // [2][2][PL]
     // longest rays first (x = N/2 has the most y steps), then plane share, then view pair
     const int per_pair = g.c * nsplit;
-    const int L = xcd_contiguous(blockIdx.x, per_pair * b.npairs);   // each XCD: consecutive rays of one pair
+    // each XCD: consecutive rays of one pair of one frame
+    const int L0 = xcd_contiguous(blockIdx.x, per_pair * b.npairs * b.nframes);
+    const int frame = L0 / (per_pair * b.npairs), L = L0 - frame * per_pair * b.npairs;
+    image += frame * b.img_stride / 4;
+    out += frame * b.ratio_stride;
+    if (MODE != FP_PLAIN) {
+        io.prompt += frame * b.sino_stride;
+        if (MODE == FP_RATIO) io.add = static_cast<const float *>(io.add) + frame * b.sino_stride;
+        else io.add = static_cast<const uint16_t *>(io.add) + frame * b.sino_stride;
+    }
+    double ll = 0.0;
     const int lp = L / per_pair, rem = L - lp * per_pair;
     const int x = g.c - rem / nsplit, split = rem % nsplit;
     const int p_lo = (int)(((long long)g.fp_np * split) / nsplit);
     const int p_hi = (int)(((long long)g.fp_np * (split + 1)) / nsplit);
     const int view = b.view0 + b.view_stride * lp;
     const int N = g.N, c = g.c;
-    if (g.cyl_lo[x] == g.cyl_hi[x]) return;
-    if (g.nzmask[view * (c + 1) + x] == 0) return;
+    if (g.cyl_lo[x] == g.cyl_hi[x] || g.nzmask[view * (c + 1) + x] == 0) {
+        if (MODE == FP_RATIO16 && io.ll_part && threadIdx.x == 0) io.ll_part[L0] = 0.0;
+        return;
+    }
     const float cosv = g.cos_psi[view], sinv = g.sin_psi[view];
 
     for (int i = threadIdx.x; i < 5 * PL; i += blockDim.x) lds[i] = f4(0.f);
@@ -298,8 +328,21 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
         if (code[t] < 0) continue;
         const int k = code[t] & 0xff, yr = (code[t] >> 8) & 0xff;
         const float sc = k == 0 ? g.y_size : g.rescalearray[(k - 1) * g.Z + yr];
-        fp_write1<RATIO>(g, mul4(acc1[t], sc), k == 0 ? 0 : 2 * k, yr, x, view, lp, out, prompt, add);
-        if (k != 0) fp_write1<RATIO>(g, mul4(acc2[t], sc), 2 * k - 1, yr, x, view, lp, out, prompt, add);
+        fp_write1<MODE>(g, mul4(acc1[t], sc), k == 0 ? 0 : 2 * k, yr, x, view, lp, out, io, ll);
+        if (k != 0) fp_write1<MODE>(g, mul4(acc2[t], sc), 2 * k - 1, yr, x, view, lp, out, io, ll);
+    }
+    if (MODE == FP_RATIO16 && io.ll_part) {
+        // workgroup sum of the log-likelihood in a fixed order (waves, then lanes)
+        for (int o = 32; o > 0; o >>= 1) ll += __shfl_xor(ll, o);
+        __syncthreads();
+        double *red = reinterpret_cast<double *>(lds);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ll;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+            io.ll_part[L0] = t;
+        }
     }
 }
 
@@ -322,22 +365,30 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
 // row N-sl) and takes entry k with readlane, so the loop over segment groups
 // waits on nothing but its own LDS reads.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBatch b, const float *__restrict__ src,
+template <typename SRC_T, bool KAHAN>
+__global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBatch b, const SRC_T *__restrict__ src,
                                                   int src_global, int apply_rescale, const int16_t *__restrict__ pseg,
                                                   float *__restrict__ bufA, float *__restrict__ bufB) {
     extern __shared__ float2 colpq[];
-    const int col = blockIdx.x + 1, role = blockIdx.y, lp = blockIdx.z;
+    const int col = blockIdx.x + 1, role = blockIdx.y, lp = blockIdx.z % b.npairs, frame = blockIdx.z / b.npairs;
     const int view = b.view0 + b.view_stride * lp;
     const int N = g.N, c = g.c, Z = g.Z, GM = g.GM, K = g.K;
+    src += frame * b.ratio_stride;
+    bufA += frame * b.buf_stride;
+    bufB += frame * b.buf_stride;
     const int lo = g.cyl_lo[col], hi = g.cyl_hi[col];
     if (lo == hi) return;
     const int gp = src_global ? view : lp;
-    const float *cp = src + ((size_t)(gp * 2 + role) * N + col) * g.spp;
+    const SRC_T *cp = src + ((size_t)(gp * 2 + role) * N + col) * g.spp;
+    auto value = [&](int i) -> float {
+        float v;
+        if constexpr (sizeof(SRC_T) == 2) v = half_at(reinterpret_cast<const uint16_t *>(cp), i);
+        else v = cp[i];
+        return apply_rescale ? v * g.bp_rescale[pseg[i]] : v;
+    };
     for (int i = threadIdx.x; i < g.bp_lds; i += blockDim.x) {
         const int s0 = g.bp_src[i], s1 = g.bp_src[i + 1];
-        float v0 = 0.0f, v1 = 0.0f;
-        if (s0 >= 0) v0 = apply_rescale ? cp[s0] * g.bp_rescale[pseg[s0]] : cp[s0];
-        if (s1 >= 0) v1 = apply_rescale ? cp[s1] * g.bp_rescale[pseg[s1]] : cp[s1];
+        const float v0 = s0 >= 0 ? value(s0) : 0.0f, v1 = s1 >= 0 ? value(s1) : 0.0f;
         colpq[i] = make_float2(v0, v1 - v0);
     }
     __syncthreads();
@@ -369,13 +420,14 @@ __global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBat
                 frac = e.w;
             }
         }
-        float i1[4], i2[4];
+        float i1[4], i2[4], c1[4], c2[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int z = lanei + 64 * j;
             const float t0 = z < Z ? colpq[z].x : 0.0f;
             i1[j] = first_even ? t0 : 0.0f;
             i2[j] = first_even ? 0.0f : t0;
+            c1[j] = c2[j] = 0.0f;
         }
         for (int k = 1; k <= K; k++) {
             const int b1 = __builtin_amdgcn_readlane(base, k - 1), b2 = __builtin_amdgcn_readlane(base, 31 + k);
@@ -390,18 +442,25 @@ __global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBat
             }
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                i1[j] = i1[j] + (a[j].x + f1 * a[j].y);
-                i2[j] = i2[j] + (bq[j].x + f2 * bq[j].y);
+                if (KAHAN) {
+                    kahan_add(i1[j], c1[j], a[j].x + f1 * a[j].y);
+                    kahan_add(i2[j], c2[j], bq[j].x + f2 * bq[j].y);
+                } else {
+                    i1[j] = i1[j] + (a[j].x + f1 * a[j].y);
+                    i2[j] = i2[j] + (bq[j].x + f2 * bq[j].y);
+                }
             }
             const int r1 = __builtin_amdgcn_readlane(range, k - 1), r2 = __builtin_amdgcn_readlane(range, 31 + k);
             const int zs1 = r1 & 0xffff, ze1 = r1 >> 16, zs2 = r2 & 0xffff;
             if (ze1 >= 192) {
                 const float2 t = q1[192];
-                i1[3] = i1[3] + (t.x + f1 * t.y);
+                if (KAHAN) kahan_add(i1[3], c1[3], t.x + f1 * t.y);
+                else i1[3] = i1[3] + (t.x + f1 * t.y);
             }
             if (zs2 + (ze1 - zs1) >= 192) {
                 const float2 t = q2[192];
-                i2[3] = i2[3] + (t.x + f2 * t.y);
+                if (KAHAN) kahan_add(i2[3], c2[3], t.x + f2 * t.y);
+                else i2[3] = i2[3] + (t.x + f2 * t.y);
             }
         }
 #pragma unroll
@@ -417,11 +476,12 @@ __global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBat
 // batch, accumulated in pair order, fused with the image update
 // (pt_cal_updateimage S3:2033-2056).  Block (64, 4): x lanes over z chunks of 4.
 // ---------------------------------------------------------------------------
+template <bool KAHAN>
 __device__ __forceinline__ float4 rotate_accumulate(const DevGeom &g, const PairBatch &b,
                                                     const float4 *__restrict__ A, const float4 *__restrict__ B,
                                                     int x, int y, int q) {
     const int N = g.N, NQ = g.NQ;
-    float4 corr = f4(0.f);
+    float4 corr = f4(0.f), comp = f4(0.f);
     for (int lp = 0; lp < b.npairs; lp++) {
         const int view = b.view0 + b.view_stride * lp;
         const float2 pnt = g.rot[((size_t)view * N + x) * N + y];
@@ -434,7 +494,15 @@ __device__ __forceinline__ float4 rotate_accumulate(const DevGeom &g, const Pair
         const size_t i3 = base + ((size_t)xi * N + yi + 1) * NQ, i4 = base + ((size_t)(xi + 1) * N + yi + 1) * NQ;
         const float4 m1 = add4(A[i1], B[i1]), m2 = add4(A[i2], B[i2]);
         const float4 m3 = add4(A[i3], B[i3]), m4 = add4(A[i4], B[i4]);
-        corr = add4(corr, add4(add4(mul4(m1, w1), mul4(m2, w2)), add4(mul4(m3, w3), mul4(m4, w4))));
+        const float4 v = add4(add4(mul4(m1, w1), mul4(m2, w2)), add4(mul4(m3, w3), mul4(m4, w4)));
+        if (KAHAN) {
+            kahan_add(corr.x, comp.x, v.x);
+            kahan_add(corr.y, comp.y, v.y);
+            kahan_add(corr.z, comp.z, v.z);
+            kahan_add(corr.w, comp.w, v.w);
+        } else {
+            corr = add4(corr, v);
+        }
     }
     return corr;
 }
@@ -447,14 +515,19 @@ __device__ __forceinline__ float update1(float img, float corr, float nf, float 
     return image_max < t ? image_max : t;   // _mm_min_ps(image_max, .)
 }
 
+template <bool KAHAN>
 __global__ void __launch_bounds__(256) rot_update_kernel(const DevGeom g, const PairBatch b,
                                                          const float4 *__restrict__ A, const float4 *__restrict__ B,
                                                          float4 *__restrict__ image, const float4 *__restrict__ nf) {
     int x, yg;
+    const int frame = blockIdx.y;
+    A += frame * b.buf_stride / 4;
+    B += frame * b.buf_stride / 4;
+    image += frame * b.img_stride / 4;
     rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
     const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
     if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
-    const float4 corr = rotate_accumulate(g, b, A, B, x, y, q);
+    const float4 corr = rotate_accumulate<KAHAN>(g, b, A, B, x, y, q);
     const size_t i = ((size_t)x * g.N + y) * g.NQ + q;
     const float4 im = image[i], n = nf[i];
     image[i] = make_float4(update1(im.x, corr.x, n.x, g.image_max), update1(im.y, corr.y, n.y, g.image_max),
@@ -463,6 +536,7 @@ __global__ void __launch_bounds__(256) rot_update_kernel(const DevGeom g, const 
 
 __device__ __forceinline__ float invert1(float s, float eps1) { return fabsf(s) > eps1 ? (1.0f / s) : 0.0f; }
 
+template <bool KAHAN>
 __global__ void __launch_bounds__(256) rot_sens_kernel(const DevGeom g, const PairBatch b,
                                                        const float4 *__restrict__ A, const float4 *__restrict__ B,
                                                        float4 *__restrict__ nf) {
@@ -470,7 +544,7 @@ __global__ void __launch_bounds__(256) rot_sens_kernel(const DevGeom g, const Pa
     rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
     const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
     if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
-    const float4 s = rotate_accumulate(g, b, A, B, x, y, q);
+    const float4 s = rotate_accumulate<KAHAN>(g, b, A, B, x, y, q);
     const int z = 4 * q;
     float4 o;
     o.x = z + 0 < g.Z ? invert1(s.x, g.eps1) : 0.0f;
@@ -487,7 +561,7 @@ __global__ void __launch_bounds__(256) rot_add_kernel(const DevGeom g, int view,
     const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
     if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
     PairBatch b{view, 0, 1};
-    const float4 r = rotate_accumulate(g, b, A, B, x, y, q);
+    const float4 r = rotate_accumulate<false>(g, b, A, B, x, y, q);
     const size_t i = ((size_t)x * g.N + y) * g.NQ + q;
     ima[i] = add4(ima[i], r);
 }
@@ -728,41 +802,114 @@ __global__ void nzmask_kernel(const DevGeom g, const int *__restrict__ colmask, 
 }
 
 // ---------------------------------------------------------------------------
+// FP64 reductions (log-likelihood, count normalisation) and FP16 conversion.
+// Every sum runs in a fixed order, so results are reproducible run to run.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum_fixed(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+    return t;   // valid in thread 0
+}
+
+__global__ void __launch_bounds__(256) ll_reduce_kernel(const double *__restrict__ part, int per, double *__restrict__ out) {
+    __shared__ double red[4];
+    const double *p = part + (size_t)blockIdx.x * per;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < per; i += blockDim.x) v += p[i];
+    const double t = block_sum_fixed(v, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+__global__ void to_half_kernel(const float *__restrict__ in, uint16_t *__restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = __half_as_ushort(__float2half_rn(in[i]));
+}
+
+// moments of one pair-column block (pair p, half h, column col): sum prompt, sum w*add, sum w*est
+__global__ void __launch_bounds__(256) count_moments_kernel(const DevGeom g, const int16_t *__restrict__ prompt,
+                                                            const float *__restrict__ add,
+                                                            const float *__restrict__ invnorm,
+                                                            const float *__restrict__ est, int view0,
+                                                            int view_stride, double *__restrict__ part) {
+    __shared__ double red[4];
+    const int col = blockIdx.x, h = blockIdx.y, lp = blockIdx.z;
+    const int view = view0 + view_stride * lp;
+    const size_t gbase = ((size_t)(view * 2 + h) * g.N + col) * g.spp;
+    const size_t ebase = ((size_t)(lp * 2 + h) * g.N + col) * g.spp;
+    double sy = 0.0, sa = 0.0, se = 0.0;
+    for (int i = threadIdx.x; i < g.spp; i += blockDim.x) {
+        const double w = invnorm[gbase + i];
+        sy += prompt[gbase + i];
+        sa += w * add[gbase + i];
+        se += w * est[ebase + i];
+    }
+    const size_t bi = ((size_t)lp * 2 + h) * g.N + col;
+    const size_t nb = (size_t)gridDim.x * gridDim.y * gridDim.z;
+    double t = block_sum_fixed(sy, red);
+    if (threadIdx.x == 0) part[bi] = t;
+    t = block_sum_fixed(sa, red);
+    if (threadIdx.x == 0) part[nb + bi] = t;
+    t = block_sum_fixed(se, red);
+    if (threadIdx.x == 0) part[2 * nb + bi] = t;
+}
+
+__global__ void __launch_bounds__(256) moments_reduce_kernel(const double *__restrict__ part, int nb,
+                                                             double *__restrict__ out) {
+    __shared__ double red[4];
+    const double *p = part + (size_t)blockIdx.x * nb;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v += p[i];
+    const double t = block_sum_fixed(v, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = out[blockIdx.x] + t;   // accumulates over calls
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 template <int MAXT, int ITEMS>
-static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float *image, float *out,
-                              const int16_t *prompt, const float *add, int ratio, int nsplit, int threads,
-                              hipStream_t s) {
-    dim3 grid(g.c * nsplit * b.npairs), block(threads);
+static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float *image, float *out, const FpIO &io,
+                              int mode, int nsplit, int threads, hipStream_t s) {
+    dim3 grid(g.c * nsplit * b.npairs * b.nframes), block(threads);
     size_t shm = sizeof(float4) * 5 * (g.Z4 + 8);
-    if (ratio)
-        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, true>), grid, block, shm, s, g, b,
-                           reinterpret_cast<const float4 *>(image), out, prompt, add, nsplit);
+    const float4 *im = reinterpret_cast<const float4 *>(image);
+    if (mode == FP_RATIO)
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO>), grid, block, shm, s, g, b, im, out, io, nsplit);
+    else if (mode == FP_RATIO16)
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO16>), grid, block, shm, s, g, b, im, out, io, nsplit);
     else
-        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, false>), grid, block, shm, s, g, b,
-                           reinterpret_cast<const float4 *>(image), out, prompt, add, nsplit);
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_PLAIN>), grid, block, shm, s, g, b, im, out, io, nsplit);
     return hipGetLastError();
 }
 
-hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *image, float *out,
-                          const int16_t *prompt, const float *add, int ratio_mode, hipStream_t s) {
-    // Launch shape: MAXT planes per thread; the planes of one ray column are split over
-    // nsplit workgroups; each workgroup has just enough 64-lane waves (at most 16) for its
-    // share of the planes and for the sampling of a row.
-    int maxt = g.fp_maxt, nsplit = g.fp_nsplit;
+// Launch shape: MAXT planes per thread; the planes of one ray column are split over
+// nsplit workgroups; each workgroup has just enough 64-lane waves (at most 16) for its
+// share of the planes and for the sampling of a row.
+void forward_shape(const DevGeom &g, int &maxt, int &nsplit, int &threads, int &items) {
+    maxt = g.fp_maxt;
+    nsplit = g.fp_nsplit;
     if (maxt < 1 || maxt > 4) maxt = 1;
     const int need = (g.fp_np + 1024 * maxt - 1) / (1024 * maxt);
     if (nsplit < need) nsplit = need;
     const int per = (g.fp_np + nsplit - 1) / nsplit;
-    int threads = (((per + maxt - 1) / maxt) + 63) / 64 * 64;
+    threads = (((per + maxt - 1) / maxt) + 63) / 64 * 64;
     if (threads < 64) threads = 64;
     if ((8 * g.NQ + threads - 1) / threads > 2) threads = ((8 * g.NQ + 1) / 2 + 63) / 64 * 64;
-    const int it2 = (8 * g.NQ + threads - 1) / threads;
+    items = (8 * g.NQ + threads - 1) / threads;
+}
+
+hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *image, float *out, const FpIO &io,
+                          int mode, hipStream_t s) {
+    int maxt, nsplit, threads, it2;
+    forward_shape(g, maxt, nsplit, threads, it2);
 #define FP_CASE(M)                                                                                    \
     case M:                                                                                           \
-        return it2 <= 1 ? fp_launch_t<M, 1>(g, b, image, out, prompt, add, ratio_mode, nsplit, threads, s) \
-                        : fp_launch_t<M, 2>(g, b, image, out, prompt, add, ratio_mode, nsplit, threads, s);
+        return it2 <= 1 ? fp_launch_t<M, 1>(g, b, image, out, io, mode, nsplit, threads, s)           \
+                        : fp_launch_t<M, 2>(g, b, image, out, io, mode, nsplit, threads, s);
     switch (maxt) {
         FP_CASE(1)
         FP_CASE(2)
@@ -773,31 +920,62 @@ hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *ima
 #undef FP_CASE
 }
 
-hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const float *src, int src_global,
-                              int apply_rescale, float *bufA, float *bufB, hipStream_t s) {
-    dim3 grid(g.N - 1, 2, b.npairs), block(64 * g.bp_waves);
+hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const void *src, int src_half, int src_global,
+                              int apply_rescale, int kahan, float *bufA, float *bufB, hipStream_t s) {
+    dim3 grid(g.N - 1, 2, b.npairs * b.nframes), block(64 * g.bp_waves);
     size_t shm = sizeof(float2) * g.bp_lds;
-    hipLaunchKernelGGL(bp_kernel, grid, block, shm, s, g, b, src, src_global, apply_rescale, g.pseg, bufA, bufB);
+    const float *sf = static_cast<const float *>(src);
+    const uint16_t *sh = static_cast<const uint16_t *>(src);
+    if (src_half) {
+        if (kahan) hipLaunchKernelGGL((bp_kernel<uint16_t, true>), grid, block, shm, s, g, b, sh, src_global, apply_rescale, g.pseg, bufA, bufB);
+        else hipLaunchKernelGGL((bp_kernel<uint16_t, false>), grid, block, shm, s, g, b, sh, src_global, apply_rescale, g.pseg, bufA, bufB);
+    } else {
+        if (kahan) hipLaunchKernelGGL((bp_kernel<float, true>), grid, block, shm, s, g, b, sf, src_global, apply_rescale, g.pseg, bufA, bufB);
+        else hipLaunchKernelGGL((bp_kernel<float, false>), grid, block, shm, s, g, b, sf, src_global, apply_rescale, g.pseg, bufA, bufB);
+    }
     return hipGetLastError();
 }
 
-static dim3 rot_grid(const DevGeom &g) { return dim3(g.N * ((g.N + 3) / 4)); }
+static dim3 rot_grid(const DevGeom &g, int nframes = 1) { return dim3(g.N * ((g.N + 3) / 4), nframes); }
 
 hipError_t launch_rotate_update(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
-                                float *image, const float *normfac, hipStream_t s) {
-    hipLaunchKernelGGL(rot_update_kernel, rot_grid(g), dim3(64, 4), 0, s, g, b,
-                       reinterpret_cast<const float4 *>(bufA), reinterpret_cast<const float4 *>(bufB),
-                       reinterpret_cast<float4 *>(image), reinterpret_cast<const float4 *>(normfac));
+                                float *image, const float *normfac, int kahan, hipStream_t s) {
+    const float4 *A = reinterpret_cast<const float4 *>(bufA), *B = reinterpret_cast<const float4 *>(bufB);
+    float4 *im = reinterpret_cast<float4 *>(image);
+    const float4 *nf = reinterpret_cast<const float4 *>(normfac);
+    if (kahan) hipLaunchKernelGGL(rot_update_kernel<true>, rot_grid(g, b.nframes), dim3(64, 4), 0, s, g, b, A, B, im, nf);
+    else hipLaunchKernelGGL(rot_update_kernel<false>, rot_grid(g, b.nframes), dim3(64, 4), 0, s, g, b, A, B, im, nf);
     return hipGetLastError();
 }
 
 hipError_t launch_rotate_sens(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
-                              float *normfac_out, hipStream_t s) {
+                              float *normfac_out, int kahan, hipStream_t s) {
     size_t n4 = (size_t)g.N * g.N * g.NQ;
     hipLaunchKernelGGL(fill_kernel, dim3(1024), dim3(256), 0, s, reinterpret_cast<float4 *>(normfac_out), n4);
-    hipLaunchKernelGGL(rot_sens_kernel, rot_grid(g), dim3(64, 4), 0, s, g, b,
-                       reinterpret_cast<const float4 *>(bufA), reinterpret_cast<const float4 *>(bufB),
-                       reinterpret_cast<float4 *>(normfac_out));
+    const float4 *A = reinterpret_cast<const float4 *>(bufA), *B = reinterpret_cast<const float4 *>(bufB);
+    float4 *o = reinterpret_cast<float4 *>(normfac_out);
+    if (kahan) hipLaunchKernelGGL(rot_sens_kernel<true>, rot_grid(g), dim3(64, 4), 0, s, g, b, A, B, o);
+    else hipLaunchKernelGGL(rot_sens_kernel<false>, rot_grid(g), dim3(64, 4), 0, s, g, b, A, B, o);
+    return hipGetLastError();
+}
+
+hipError_t launch_ll_reduce(const double *part, int per_frame, int nframes, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(ll_reduce_kernel, dim3(nframes), dim3(256), 0, s, part, per_frame, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_half(const float *in, uint16_t *out, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(to_half_kernel, dim3(2048), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_moments(const DevGeom &g, const int16_t *prompt, const float *add, const float *invnorm,
+                                const float *est, int view0, int view_stride, int npairs, double *part,
+                                double *out, hipStream_t s) {
+    dim3 grid(g.N, 2, npairs);
+    hipLaunchKernelGGL(count_moments_kernel, grid, dim3(256), 0, s, g, prompt, add, invnorm, est, view0, view_stride,
+                       part);
+    hipLaunchKernelGGL(moments_reduce_kernel, dim3(3), dim3(256), 0, s, part, (int)(grid.x * grid.y * grid.z), out);
     return hipGetLastError();
 }
 

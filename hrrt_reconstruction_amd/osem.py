@@ -159,6 +159,61 @@ class Osem3D(Layout):
     def end_iteration(self, image):
         _lib.call("hrrt_end_iteration", self.ctx, _p(image), _stream())
 
+    # ---- batched dynamic frames ----
+    def osem_subset_frames(self, images, normfac, prompts, adds, isubset: int):
+        """images [F][X][Y][Z4], prompts/adds [F][views/2][2][N][spp]: one subset for every frame."""
+        _lib.call("hrrt_osem_subset_frames", self.ctx, int(images.shape[0]), _p(images), _p(normfac[isubset]),
+                  _p(prompts), _p(adds), int(isubset), _stream())
+
+    def osem_iteration_frames(self, images, normfac, prompts, adds):
+        """One iteration (all subsets, graph-replayed, then the end-of-iteration threshold) for every frame."""
+        s = torch.cuda.current_stream()
+        if s.cuda_stream == 0:
+            raise ValueError("graph replay needs a non-default stream (use torch.cuda.stream(...))")
+        _lib.call("hrrt_osem_iteration_frames", self.ctx, int(images.shape[0]), _p(images), _p(normfac),
+                  _p(prompts), _p(adds), C.c_void_p(s.cuda_stream))
+
+    # ---- precision form: FP16 per-bin inputs, Kahan accumulation, FP64 log-likelihood ----
+    @staticmethod
+    def to_half(t):
+        """float32 device tensor -> FP16 (round to nearest even) as an int16 tensor of the FP16 bits."""
+        out = torch.empty(t.shape, dtype=torch.int16, device=t.device)
+        _lib.call("hrrt_to_half", _p(t), _p(out), int(t.numel()), _stream())
+        return out
+
+    def calculate_normfac_mixed(self, invnorm16):
+        nf = torch.zeros((self.subsets, self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)
+        _lib.call("hrrt_calculate_normfac_mixed", self.ctx, _p(invnorm16), _p(nf), _stream())
+        return nf
+
+    def osem_subset_mixed(self, image, normfac, prompt, add16, isubset: int, invnorm16=None, loglik=None):
+        """One subset of the precision form; loglik: float64 device tensor of 1 element (or None)."""
+        _lib.call("hrrt_osem_subset_mixed", self.ctx, _p(image), _p(normfac[isubset]), _p(prompt), _p(add16),
+                  _p(invnorm16), int(isubset), _p(loglik), _stream())
+
+    def osem_subset_frames_mixed(self, images, normfac, prompts, adds16, isubset: int, invnorm16=None, loglik=None):
+        _lib.call("hrrt_osem_subset_frames_mixed", self.ctx, int(images.shape[0]), _p(images), _p(normfac[isubset]),
+                  _p(prompts), _p(adds16), _p(invnorm16), int(isubset), _p(loglik), _stream())
+
+    def osem_iteration_frames_mixed(self, images, normfac, prompts, adds16, invnorm16=None, loglik=None):
+        """loglik: float64 device tensor [subsets][frames] (or None)."""
+        s = torch.cuda.current_stream()
+        if s.cuda_stream == 0:
+            raise ValueError("graph replay needs a non-default stream (use torch.cuda.stream(...))")
+        _lib.call("hrrt_osem_iteration_frames_mixed", self.ctx, int(images.shape[0]), _p(images), _p(normfac),
+                  _p(prompts), _p(adds16), _p(invnorm16), _p(loglik), C.c_void_p(s.cuda_stream))
+
+    def count_moments(self, image, prompt, add, invnorm):
+        """FP64 (sum prompts, sum w*add, sum w*est) over the whole sinogram, w = 1/(N*A)."""
+        out = torch.zeros(3, dtype=torch.float64, device=self.device)
+        _lib.call("hrrt_count_moments", self.ctx, _p(image), _p(prompt), _p(add), _p(invnorm), _p(out), _stream())
+        return out
+
+    def count_scale(self, image, prompt, add, invnorm):
+        """Global count normalisation: the factor that makes the expected total counts of image match the data."""
+        m = self.count_moments(image, prompt, add, invnorm).cpu().numpy()
+        return float((m[0] - m[1]) / m[2])
+
     def calculate_osem3d(self, image, normfac, prompt, add, iterations: int = 1):
         for _ in range(iterations):
             for i in range(self.subsets):

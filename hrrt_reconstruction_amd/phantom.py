@@ -100,17 +100,21 @@ def gaussian_blur_radial(x, sigma_bins):
 
 def make_study(proj_file, rings, span, maxdel, def_elements, def_angles, bin_mm, plane_sep, ring_radius,
                total_counts, gen, device, with_scatter=False, norm_sigma=0.1, randoms_frac=0.10,
-               scatter_frac=0.15):
+               scatter_frac=0.15, norm=None, atten=None):
     """Build file-layout sinograms for an already forward-projected phantom.
 
     proj_file: (P x) in file layout [S][def_angles][def_elements] (float32, device).
     Returns dict(prompt=int16, delayed=f32, norm=f32, atten=f32, scatter=f32 or None, alpha=float).
+    norm / atten: reuse the factors of an earlier frame of the same study (a dynamic study has one
+    normalisation and one attenuation map).
     """
     segs = file_segments(rings, span, maxdel)
     ring_spacing = 2.0 * plane_sep
     shape = proj_file.shape
-    norm = torch.exp(norm_sigma * torch.randn(shape, generator=gen, device=device, dtype=torch.float32))
-    atten = atten_file(def_elements, def_angles, bin_mm, segs, ring_radius, ring_spacing, span, device)
+    if norm is None:
+        norm = torch.exp(norm_sigma * torch.randn(shape, generator=gen, device=device, dtype=torch.float32))
+    if atten is None:
+        atten = atten_file(def_elements, def_angles, bin_mm, segs, ring_radius, ring_spacing, span, device)
     base = proj_file / (norm * atten)                 # trues per unit activity
     sfrac = scatter_frac if with_scatter else 0.0
     tfrac = 1.0 - randoms_frac - sfrac
@@ -141,3 +145,21 @@ def estimates_to_file(estimates_by_view, S, views, N, device):
         out[:, v, :] = e[:S]
         out[:, v + views // 2, :] = e[S:]
     return out
+
+
+def dynamic_frames(nframes, counts_lo, counts_hi, gen):
+    """Frame durations (s), mid times (min), total counts (log-uniform in [counts_lo, counts_hi]) and
+    per-sphere washout rates (1/min) of a seeded dynamic study: sphere k has activity
+    1 + 3 exp(-rate_k t) at frame mid time t, the background cylinder stays at 1."""
+    base = [15, 30, 60, 120, 300, 600]
+    per = max(1, nframes // len(base))
+    dur = [base[min(i // per, len(base) - 1)] for i in range(nframes)]
+    t, mids = 0.0, []
+    for d in dur:
+        mids.append((t + d / 2) / 60.0)
+        t += d
+    u = torch.rand(nframes, generator=gen)
+    counts = [float(math.exp(math.log(counts_lo) + (math.log(counts_hi) - math.log(counts_lo)) * float(v))) for v in u]
+    r = torch.rand(6, generator=gen)
+    rates = [float(math.exp(math.log(0.005) + (math.log(0.2) - math.log(0.005)) * float(v))) for v in r]
+    return dur, mids, counts, rates

@@ -143,6 +143,61 @@ int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subse
                      const int16_t *d_prompt, const float *d_add, int isubset, void *stream);
 int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream);           /* S3:4288-4297           */
 
+/* ---------------- batched dynamic frames (no counterpart in the reference) ----------------
+ * The reference reconstructs one frame per process run; a dynamic study runs the
+ * same subset update (hrrt_osem_subset) for every frame.  These entry points do
+ * it for nframes frames at once, one launch per kernel, frames at fixed strides:
+ *   d_images  [nframes][X][Y][Z4]            (stride hrrt_dims.image_floats)
+ *   d_prompts [nframes][pair-column] int16   (stride hrrt_dims.paircol_elems)
+ *   d_adds    [nframes][pair-column] float   (stride hrrt_dims.paircol_elems)
+ * The sensitivity images are shared (same geometry, norm and attenuation).  Each
+ * frame's result is bit-identical to hrrt_osem_subset on that frame alone.
+ * Per-frame scratch (~2 rotated-frame volumes per view pair per frame) is kept by
+ * the context. */
+int hrrt_osem_subset_frames(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac_subset,
+                            const int16_t *d_prompts, const float *d_adds, int isubset, void *stream);
+/* One full iteration (every subset in sequence order, S3:4174-4245, then the
+ * end-of-iteration threshold S3:4288-4297) for all frames.  The subsets' launches
+ * are captured once into one HIP graph per subset and replayed; the graphs are
+ * rebuilt when nframes, a buffer pointer, the stream or a launch shape changes.
+ * d_normfac: [subsets][X][Y][Z4].  `stream` must not be the default stream. */
+int hrrt_osem_iteration_frames(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac,
+                               const int16_t *d_prompts, const float *d_adds, void *stream);
+
+/* ---------------- precision form (no counterpart in the reference) ----------------
+ * The reference keeps every sinogram in float32 and sums in float32.  This form
+ * stores the per-bin float inputs of the hot loop -- the additive term
+ * (d*N + s)*A and the sensitivity weights 1/(N*A) -- as IEEE FP16 (bits in
+ * uint16_t, round to nearest even), keeps prompts exact int16, computes in FP32,
+ * accumulates the back projection over segment groups and the rotation over view
+ * pairs with compensated (Kahan) summation, and uses FP64 only for the Poisson
+ * log-likelihood of each subset and the global count moments.  Results are
+ * within tolerance of an FP64 projector, not bit-identical to the FP32 path. */
+int hrrt_to_half(const float *d_in, uint16_t *d_out, long long n, void *stream);
+/* CalculateNormfac (S3:3760) from FP16 1/(N*A), Kahan accumulation. */
+int hrrt_calculate_normfac_mixed(hrrt_ctx *ctx, const uint16_t *d_invnorm16, float *d_normfac, void *stream);
+/* hrrt_osem_subset with FP16 additive term.  When d_loglik is not NULL, *d_loglik
+ * (device double) receives sum over the subset's bins of y*log(mu) - mu with
+ * mu = (est + add) * w, w = FP16 1/(N*A) (d_invnorm16, shared by all frames), at the
+ * image before the update, summed in FP64 in a fixed order. */
+int hrrt_osem_subset_mixed(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subset, const int16_t *d_prompt,
+                           const uint16_t *d_add16, const uint16_t *d_invnorm16, int isubset, double *d_loglik,
+                           void *stream);
+/* The same for nframes frames (d_loglik: [nframes] or NULL). */
+int hrrt_osem_subset_frames_mixed(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac_subset,
+                                  const int16_t *d_prompts, const uint16_t *d_adds16, const uint16_t *d_invnorm16,
+                                  int isubset, double *d_loglik, void *stream);
+/* Graph-replayed iteration of the precision form (d_loglik: [subsets][nframes] or NULL). */
+int hrrt_osem_iteration_frames_mixed(hrrt_ctx *ctx, int nframes, float *d_images, const float *d_normfac,
+                                     const int16_t *d_prompts, const uint16_t *d_adds16,
+                                     const uint16_t *d_invnorm16, double *d_loglik, void *stream);
+/* Global count moments in FP64 over the whole sinogram, forward projecting d_image
+ * for every view pair: d_out[0] = sum prompts, d_out[1] = sum w*add,
+ * d_out[2] = sum w*est, w = 1/(N*A).  The count normalisation of an image is
+ * (d_out[0] - d_out[1]) / d_out[2]. */
+int hrrt_count_moments(hrrt_ctx *ctx, const float *d_image, const int16_t *d_prompt, const float *d_add,
+                       const float *d_invnorm, double *d_out, void *stream);
+
 /* Kernel timing of hrrt_osem_subset (no counterpart in the reference, whose
  * only timing is clock() around loops, S3:4219-4285).  When enabled, HIP
  * events are recorded on the launch stream around the three kernels of every

@@ -251,3 +251,46 @@ def rotate(G: Geometry, buf, view):
         ok = (np.trunc(xx) >= 0) & (np.trunc(xx) <= N - 2) & (np.trunc(yy) >= 0) & (np.trunc(yy) <= N - 2)
         out[x, ys[ok]] = bilinear(b, xx[ok], yy[ok])
     return out
+
+
+def gather_pair(sino, view):
+    """[S][views][N] -> [2S][N] rows of view and view + views/2."""
+    V = sino.shape[1]
+    return np.concatenate([sino[:, view], sino[:, view + V // 2]], 0)
+
+
+def sensitivity(G: Geometry, invnorm, subset, subsets):
+    """FP64 sum over the subset's view pairs of the rotated back projection of 1/(N*A)."""
+    out = np.zeros((G.N, G.N, G.Z4))
+    for j in range(G.views // subsets // 2):
+        v = subset + subsets * j
+        out += rotate(G, back_rotframe(G, gather_pair(invnorm, v).astype(np.float64), v), v)
+    return out
+
+
+def invert(G: Geometry, sens, eps1):
+    out = np.zeros_like(sens)
+    for x in range(G.N):
+        lo, hi = G.cyl[x]
+        s = sens[x, lo:hi, :G.Z]
+        out[x, lo:hi, :G.Z] = np.where(np.abs(s) > eps1, 1.0 / np.where(s == 0, 1, s), 0.0)
+    return out
+
+
+def osem_subset(G: Geometry, image, normfac, add, prompt, subset, subsets, eps3=1e-10, image_max=1e4):
+    """FP64 OSEM subset update with the OP ratio and the clamped multiplicative update."""
+    corr = np.zeros((G.N, G.N, G.Z4))
+    for j in range(G.views // subsets // 2):
+        v = subset + subsets * j
+        est = forward(G, image, v)
+        a = gather_pair(add, v).astype(np.float64)
+        y = gather_pair(prompt, v).astype(np.float64)
+        den = est + a
+        ratio = np.where(y == 0, 0.0, np.where(den > eps3, y / np.where(den > eps3, den, 1.0), 1.0))
+        corr += rotate(G, back_rotframe(G, ratio, v), v)
+    out = image.copy()
+    for x in range(G.N):
+        lo, hi = G.cyl[x]
+        c = np.clip(corr[x, lo:hi], 0.01, 100.0)
+        out[x, lo:hi] = np.minimum(image_max, image[x, lo:hi] * (c * normfac[x, lo:hi]))
+    return out
