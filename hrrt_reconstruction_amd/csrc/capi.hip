@@ -48,6 +48,7 @@ struct hrrt_ctx {
     uint8_t *gapmask = nullptr;      // [th][def_angles][def_elements]
     float *h_img = nullptr, *h_prj = nullptr;  // device staging for the host shims
     int device = 0;
+    int bp_incremental = 0;          // back projector of the FP32 subset path: 0 direct, 1 incremental
     // optional per-kernel timing of hrrt_osem_subset (HIP events on the launch stream)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -383,6 +384,84 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
             }
         }
 
+    // incremental back projector (back_proj3d_view_n1thread PJ:4162-4668): one lane's column in
+    // LDS (segment 0, then segments 2k and 2k-1 each with a zero guard plane on both sides,
+    // PJ:1689-1692), the per-group difference factors, the first-row lookups (rows c-1, c+1)
+    // and the breakpoint schedule shared by every column (rows c-2 down to 1)
+    std::vector<int2> ibase(G.K + 1, make_int2(0, 0));
+    std::vector<int> isrc, iconv;
+    for (int z = 0; z < Z; z++) {
+        isrc.push_back(G.seg_data[0] + z);
+        iconv.push_back(0);
+    }
+    for (int k = 1; k <= G.K; k++) {
+        int bases[2];
+        for (int e = 0; e < 2; e++) {
+            const int sg = e == 0 ? 2 * k : 2 * k - 1, yb = G.yr_bottom[sg], yt = G.yr_top[sg];
+            bases[e] = (int)isrc.size() - (yb - 1);
+            for (int yr = yb - 1; yr <= yt + 1; yr++) {
+                isrc.push_back(yr >= yb && yr <= yt ? G.seg_data[sg] + (yr - yb) : -1);
+                iconv.push_back(yr <= yt ? k : 0);
+            }
+        }
+        ibase[k] = make_int2(bases[0], bases[1]);
+    }
+    const int inc_lds = (int)isrc.size();
+    isrc.push_back(-1);
+    iconv.push_back(0);
+    std::vector<float> itn(G.K + 1, 0.f);
+    std::vector<int> ifirst((size_t)(G.K + 1) * 12, 0);
+    for (int k = 1; k <= G.K; k++) {
+        const int th = 2 * k;
+        itn[k] = -G.sin_theta[th] / G.cos_theta[th];
+        const size_t l1 = (size_t)(c - 1) * GM + k - 1, l2 = (size_t)(c + 1) * GM + k - 1;
+        const hrrt::Geometry::Blut &bc = G.blut[l2];
+        int *f = &ifirst[(size_t)k * 12];
+        f[0] = G.bl_zsa[l1]; f[1] = G.bl_zea[l1]; f[2] = G.bl_yra[l1]; f[3] = bits(G.bl_frac[l1]);
+        f[4] = G.bl_zsa[l2]; f[5] = G.bl_yra[l2]; f[6] = bits(G.bl_frac[l2]);
+        f[7] = bc.z_start; f[8] = bc.z_end; f[9] = bc.yr_start;
+    }
+    std::vector<int4> isched;
+    std::vector<int2> irow(c + 1, make_int2(0, 0));
+    // every difference-column read must stay inside its segment's window [yb-1, yt+1], and
+    // one thread per plane needs Z <= 256
+    int inc_ok = Z <= 256;
+    auto in_window = [&](int k, int first, int count) {      // rows first .. first+count of group k
+        if (count < 0) return true;
+        for (int sg = 2 * k - 1; sg <= 2 * k; sg++)
+            if (first < G.yr_bottom[sg] - 1 || first + count > G.yr_top[sg] + 1) return false;
+        return true;
+    };
+    for (int k = 1; k <= G.K; k++) {
+        const int *f = &ifirst[(size_t)k * 12];
+        const int n = f[1] - f[0] + 1;
+        if (G.yr_bottom[2 * k] != G.yr_bottom[2 * k - 1] || G.yr_top[2 * k] != G.yr_top[2 * k - 1]) inc_ok = 0;
+        if (n > 0 && (!in_window(k, f[2], n) || !in_window(k, f[5], n))) inc_ok = 0;
+        if (!in_window(k, f[9], f[8] - f[7])) inc_ok = 0;
+    }
+    for (int y = c - 2; y >= 1; y--) {
+        const int off = (int)isched.size() / 2;
+        for (int k = 1; k <= G.K; k++) {
+            const size_t li = (size_t)y * GM + k - 1;
+            if (G.blut_flag[li] == 0) continue;
+            const hrrt::Geometry::Blut &ba = G.blut[li], &bb = G.blut[(size_t)(N - y) * GM + k - 1];
+            if (ba.z_start - ba.z_start_pre != -1) inc_ok = 0;   // the range moved by other than one plane
+            if (ba.z_end_pre < ba.z_start_pre || !in_window(k, ba.yr_start_pre, ba.z_end_pre - ba.z_start_pre))
+                inc_ok = 0;
+            const int th = 2 * k;
+            float tn = -G.sin_theta[th] / G.cos_theta[th];
+            tn = 1 - tn * (float)ba.num - ba.coef_pre;                    // PJ:4371-4373
+            const float coef1 = -tn * G.cos_theta[th] / G.sin_theta[th];
+            isched.push_back(make_int4(k, bits(coef1), ba.z_start_pre, bb.z_start_pre));
+            isched.push_back(make_int4(ba.yr_start_pre, ba.z_start, ba.z_end_pre, 0));
+        }
+        irow[y] = make_int2(off, (int)isched.size() / 2 - off);
+    }
+    if (isched.empty()) isched.push_back(make_int4(0, 0, 0, 0));
+    d.inc_ok = inc_ok;
+    d.inc_lds = inc_lds;
+    if ((size_t)inc_lds * sizeof(float) > 64 * 1024) d.inc_ok = 0;
+
     int r = 0;
     if ((r = ctx->upload(G.cyl_lo, d.cyl_lo)) || (r = ctx->upload(G.cyl_hi, d.cyl_hi)) ||
         (r = ctx->upload(G.sin_psi, d.sin_psi)) || (r = ctx->upload(G.cos_psi, d.cos_psi)) ||
@@ -392,7 +471,10 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         (r = ctx->upload(resc, d.rescalearray)) || (r = ctx->upload(blut, d.bp_lut)) ||
         (r = ctx->upload(rot, d.rot)) || (r = ctx->upload(fplanes, d.fp_planes)) || (r = ctx->upload(pseg, d.pseg)) ||
         (r = ctx->upload(plane_seg, d.plane_seg)) || (r = ctx->upload(bp_src, d.bp_src)) ||
-        (r = ctx->upload(blut2, d.bp_lut2))) {
+        (r = ctx->upload(blut2, d.bp_lut2)) || (r = ctx->upload(isrc, d.inc_src)) ||
+        (r = ctx->upload(iconv, d.inc_conv)) || (r = ctx->upload(itn, d.inc_tn)) ||
+        (r = ctx->upload(ibase, d.inc_base)) || (r = ctx->upload(ifirst, d.inc_first)) ||
+        (r = ctx->upload(isched, d.inc_sched)) || (r = ctx->upload(irow, d.inc_row))) {
         delete ctx;
         return r;
     }
@@ -547,7 +629,10 @@ extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_no
     io.add = d_add;
     HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, io, hrrt::FP_RATIO, S(stream)));
     ctx->mark(1, S(stream));
-    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->sub_ratio, 0, 0, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    if (ctx->bp_incremental)
+        HIP_TRY(hrrt::launch_backproject_incremental(ctx->dg, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    else
+        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->sub_ratio, 0, 0, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
     ctx->mark(2, S(stream));
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, 0, S(stream)));
     ctx->mark(-1, S(stream));
@@ -593,7 +678,10 @@ static int frames_subset_launch(hrrt_ctx *ctx, int nframes, float *d_images, con
         hrrt::forward_shape(ctx->dg, maxt, nsplit, threads, items);
         HIP_TRY(hrrt::launch_ll_reduce(ctx->ll_part, ctx->dg.c * nsplit * b.npairs, nframes, pr.loglik, s));
     }
-    HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->f_ratio, 0, 0, 0, pr.mixed, ctx->fA, ctx->fB, s));
+    if (ctx->bp_incremental && !pr.mixed)
+        HIP_TRY(hrrt::launch_backproject_incremental(ctx->dg, b, ctx->f_ratio, 0, 0, ctx->fA, ctx->fB, s));
+    else
+        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->f_ratio, 0, 0, 0, pr.mixed, ctx->fA, ctx->fB, s));
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->fA, ctx->fB, d_images, d_normfac_subset, pr.mixed, s));
     return HRRT_OK;
 }
@@ -808,6 +896,28 @@ extern "C" int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int v
     HIP_TRY(hrrt::launch_estimate_to_paircol(ctx->dg, d_prj, ctx->pc1, S(stream)));
     HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->pc1, 0, 0, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
     HIP_TRY(hrrt::launch_rotate_add(ctx->dg, view, ctx->bufA, ctx->bufB, d_ima, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream) {
+    if (!ctx || !d_prj || !d_ima) return fail(HRRT_ERR_ARG, "null argument");
+    if (view < 0 || view >= ctx->geo.views / 2) return fail(HRRT_ERR_ARG, "view %d out of range", view);
+    if (!ctx->dg.inc_ok) return fail(HRRT_ERR_UNSUPPORTED, "incremental back projection is not available for this geometry");
+    int r;
+    if ((r = ctx->ensure_pair_scratch())) return r;
+    hrrt::PairBatch b{view, 0, 1};
+    HIP_TRY(hrrt::launch_estimate_to_paircol(ctx->dg, d_prj, ctx->pc1, S(stream)));
+    HIP_TRY(hrrt::launch_backproject_incremental(ctx->dg, b, ctx->pc1, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    HIP_TRY(hrrt::launch_rotate_add(ctx->dg, view, ctx->bufA, ctx->bufB, d_ima, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_set_backproject_algorithm(hrrt_ctx *ctx, int incremental) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    if (incremental && !ctx->dg.inc_ok)
+        return fail(HRRT_ERR_UNSUPPORTED, "incremental back projection is not available for this geometry");
+    ctx->bp_incremental = incremental ? 1 : 0;
+    ctx->drop_graphs();
     return HRRT_OK;
 }
 

@@ -33,6 +33,16 @@ struct DevGeom {
     const int *bp_src;                   // [bp_lds + 1] LDS entry -> pair-column plane (-1 = zero)
     int bp_lds;                          // entries of the zero-padded back-projection column
     int8_t *nzmask;                      // [views/2][N/2+1], 0 = masked column
+    // incremental back projector (back_proj3d_view_n1thread PJ:4162-4668)
+    int inc_ok;                          // 1 when every breakpoint of the geometry is of the kind handled
+    int inc_lds;                         // LDS floats of one lane's column (segments with guard planes)
+    const int *inc_src;                  // [inc_lds] LDS entry -> pair-column plane (-1 = zero)
+    const int *inc_conv;                 // [inc_lds] group k whose tan converts the entry to a difference (0 = none)
+    const float *inc_tn;                 // [K+1] -sin(theta)/cos(theta) of segment 2k
+    const int2 *inc_base;                // [K+1] LDS index of plane 0 of segments 2k, 2k-1
+    const int *inc_first;                // [K+1][12] first-row lookups (rows c-1, c+1) and breakpoint of row c+1
+    const int4 *inc_sched;               // [2*n] breakpoints, rows c-2 down to 1, groups ascending
+    const int2 *inc_row;                 // [c+1] first entry and count of row y
     int fp_maxt, fp_nsplit;              // forward-projector launch shape (tasks/thread, workgroups/(pair,x))
     int bp_waves;                        // back-projector waves per workgroup
 };
@@ -68,6 +78,9 @@ void forward_shape(const DevGeom &g, int &maxt, int &nsplit, int &threads, int &
 // src: float pair-column, or FP16 bits when src_half; kahan: compensated accumulation over groups
 hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const void *src_paircol, int src_half,
                               int src_global, int apply_rescale, int kahan, float *bufA, float *bufB, hipStream_t s);
+// incremental back projection of every view pair of the batch into the rotated-frame halves
+hipError_t launch_backproject_incremental(const DevGeom &g, const PairBatch &b, const float *src, int src_global,
+                                          int apply_rescale, float *bufA, float *bufB, hipStream_t s);
 hipError_t launch_rotate_update(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
                                 float *image, const float *normfac, int kahan, hipStream_t s);
 hipError_t launch_rotate_sens(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,

@@ -18,9 +18,6 @@ __device__ __forceinline__ float4 f4(float a) { return make_float4(a, a, a, a); 
 __device__ __forceinline__ float4 add4(float4 a, float4 b) {
     return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
-__device__ __forceinline__ float4 sub4(float4 a, float4 b) {
-    return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
-}
 __device__ __forceinline__ float4 mul4(float4 a, float s) {
     return make_float4(a.x * s, a.y * s, a.z * s, a.w * s);
 }
@@ -398,6 +395,14 @@ __global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBat
     dst += (size_t)lp * N * N * g.Z4;
     const int lanei = threadIdx.x & 63, nw = blockDim.x >> 6;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (lo == c && hi == c + 1) {
+        // a one-pixel column: the incremental variant also writes rows c-1 and c+1, zero here
+        for (int t = threadIdx.x; t < 2 * 64; t += blockDim.x) {
+            const int V = (t >> 6) ? c + 1 : c - 1;
+            const int X = role == 0 ? col : V, Y = role == 0 ? V : col;
+            for (int z = t & 63; z < Z; z += 64) dst[((size_t)X * N + Y) * g.Z4 + z] = 0.0f;
+        }
+    }
     for (int Vv = lo + wave; Vv < hi; Vv += nw) {
         const int V = __builtin_amdgcn_readfirstlane(Vv);
         const int X = role == 0 ? col : V, Y = role == 0 ? V : col;
@@ -467,6 +472,200 @@ __global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBat
         for (int j = 0; j < 4; j++) {
             const int z = lanei + 64 * j;
             if (z < Z) o[z] = i1[j] + i2[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Back projector, incremental variant (back_proj3d_thread1 PJ:1622-1699 with
+// back_proj3d_view_n1thread PJ:4162-4668), the reference's default.  Walking a
+// column's rows away from the centre, each plane keeps running sums s1 (row y)
+// and s2 (row N-y) and their per-row increments d1, d2; at the rows where a
+// group's integer shift changes (the breakpoints of blutacc, PJ:718-784) the
+// increments are corrected.  One workgroup per (column x <= N/2, lane, view
+// pair); lane l is bin x or N-x (l & 1) of view v or v+90 (l >> 1); a thread
+// owns one plane and applies, to its own accumulators, exactly the terms the
+// reference's loops apply to that plane, in the reference's order (the order
+// of two terms of one breakpoint depends only on the breakpoint, so it is
+// wave-uniform).  Lanes 0/1 write the rotated-frame half A, lanes 2/3 the half
+// B, so each voxel of A and of B has one writer and R = A + B as for the
+// direct variant.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) bp_inc_kernel(const DevGeom g, const PairBatch b, const float *__restrict__ src,
+                                                     int src_global, int apply_rescale,
+                                                     const int16_t *__restrict__ pseg, float *__restrict__ bufA,
+                                                     float *__restrict__ bufB) {
+    extern __shared__ float P[];
+    const int x = blockIdx.x + 1, l = blockIdx.y, lp = blockIdx.z % b.npairs, frame = blockIdx.z / b.npairs;
+    const int N = g.N, c = g.c, Z = g.Z, K = g.K;
+    if (x == c && (l & 1)) return;          // at x == N/2 lanes 1/3 duplicate lanes 0/2
+    const int lo = g.cyl_lo[x], hi = g.cyl_hi[x];
+    if (lo == hi) return;
+    const int view = b.view0 + b.view_stride * lp;
+    src += frame * b.ratio_stride;
+    const int h = l >> 1, col = (l & 1) ? N - x : x;
+    const int gp = src_global ? view : lp;
+    const float *cp = src + ((size_t)(gp * 2 + h) * N + col) * g.spp;
+    for (int i = threadIdx.x; i < g.inc_lds; i += blockDim.x) {   // transpose PJ:1654-1693
+        const int si = g.inc_src[i];
+        float v = 0.0f;
+        if (si >= 0) v = apply_rescale ? cp[si] * g.bp_rescale[pseg[si]] : cp[si];
+        P[i] = v;
+    }
+    __syncthreads();
+    const int z = threadIdx.x;
+    const bool act = z < Z;
+    float *dst = (l < 2 ? bufA : bufB) + frame * b.buf_stride + (size_t)lp * N * N * g.Z4;
+    // s1 of row y goes to A(col, y) / B(N-y, col), s2 to A(col, N-y) / B(y, col)
+    auto vox = [&](int y, bool second) -> float * {
+        const int X = l < 2 ? col : (second ? y : N - y);
+        const int Y = l < 2 ? (second ? N - y : y) : col;
+        return dst + ((size_t)X * N + Y) * g.Z4;
+    };
+    const int ylast = lo < c - 1 ? lo : c - 1;                       // rows c-1 .. ylast are written
+    if (g.nzmask[view * (c + 1) + x] == 0) {                          // masked column PJ:4236-4262
+        if (act) {
+            vox(c, false)[z] = 0.0f;
+            for (int y = c - 1; y >= ylast; y--) {
+                vox(y, false)[z] = 0.0f;
+                vox(y, true)[z] = 0.0f;
+            }
+        }
+        return;
+    }
+    // centre voxels PJ:4265-4285
+    if (act) {
+        float sv = P[z];
+        for (int k = 1; k <= K; k++) {
+            const int2 bs = g.inc_base[k];
+            if (z >= g.yb[2 * k] && z <= g.yt[2 * k]) sv = sv + (P[bs.x + z] + P[bs.y + z]);
+        }
+        vox(c, false)[z] = sv;
+    }
+    // first row y = c-1 by direct interpolation PJ:4287-4312
+    float s1 = act ? P[z] : 0.0f, s2 = s1, d1 = 0.0f, d2 = 0.0f;
+    for (int k = 1; k <= K; k++) {
+        const int *f = g.inc_first + k * 12;
+        const int2 bs = g.inc_base[k];
+        const int n = f[1] - f[0] + 1;
+        const float c1 = __int_as_float(f[3]), c2 = __int_as_float(f[6]);
+        const int ia = z - f[0], ib = z - f[4];
+        const bool va = act && ia >= 0 && ia < n, vb = act && ib >= 0 && ib < n;
+        float a1 = 0.0f, a2 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+        if (va) {
+            const int yr = f[2] + ia;
+            float p = P[bs.x + yr], q = P[bs.x + yr + 1];
+            a1 = p + c1 * (q - p);
+            p = P[bs.y + yr]; q = P[bs.y + yr + 1];
+            a2 = p + c1 * (q - p);
+        }
+        if (vb) {
+            const int yr = f[5] + ib;
+            float p = P[bs.y + yr], q = P[bs.y + yr + 1];
+            b1 = p + c2 * (q - p);
+            p = P[bs.x + yr]; q = P[bs.x + yr + 1];
+            b2 = p + c2 * (q - p);
+        }
+        if (f[4] <= f[0]) {          // iteration ia <= ib: the row-y term first
+            if (va) { s1 = s1 + a1; s2 = s2 + a2; }
+            if (vb) { s1 = s1 + b1; s2 = s2 + b2; }
+        } else {
+            if (vb) { s1 = s1 + b1; s2 = s2 + b2; }
+            if (va) { s1 = s1 + a1; s2 = s2 + a2; }
+        }
+    }
+    __syncthreads();
+    // the columns become scaled differences, in place PJ:4313-4330
+    for (int base = 0; base < g.inc_lds; base += 4 * (int)blockDim.x) {
+        float nv[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i = base + threadIdx.x + j * blockDim.x;
+            nv[j] = 0.0f;
+            if (i < g.inc_lds) {
+                const int k = g.inc_conv[i];
+                nv[j] = k ? (P[i + 1] - P[i]) * g.inc_tn[k] : P[i];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i = base + threadIdx.x + j * blockDim.x;
+            if (i < g.inc_lds) P[i] = nv[j];
+        }
+        __syncthreads();
+    }
+    for (int k = 1; k <= K; k++) {                                   // PJ:4325-4335
+        const int *f = g.inc_first + k * 12;
+        const int2 bs = g.inc_base[k];
+        if (act && z >= g.yb[2 * k] - 1 && z <= g.yt[2 * k]) {
+            d1 = d1 + P[bs.x + z];
+            d2 = d2 + P[bs.y + z];
+        }
+        const int j = z - f[7];
+        if (act && j >= 0 && j <= f[8] - f[7]) {
+            const int yr = f[9] + j;
+            d1 = d1 - P[bs.y + yr];
+            d2 = d2 - P[bs.x + yr];
+        }
+    }
+    if (act) {
+        vox(c - 1, false)[z] = s1;
+        vox(c - 1, true)[z] = s2;
+    }
+    // rows c-2 .. ystart PJ:4337-4562: breakpoints (the range moved down by one plane), then
+    // the running sums advance by their increments
+    for (int y = c - 2; y >= lo; y--) {
+        const int2 rr = g.inc_row[y];
+        for (int e = rr.x; e < rr.x + rr.y; e++) {
+            const int4 q0 = g.inc_sched[2 * e], q1 = g.inc_sched[2 * e + 1];
+            const int k = q0.x, zp = q0.z, z2p = q0.w, yp = q1.x, z3 = q1.y, zep = q1.z;
+            const float cf = __int_as_float(q0.y);
+            const int2 bs = g.inc_base[k];
+            const float *D1 = P + bs.x, *D2 = P + bs.y;
+            const float D1p = D1[yp], D2p = D2[yp];
+            if (act && z == z3) {
+                s1 = s1 - D1p * cf; s2 = s2 - D2p * cf; d1 = d1 + D1p; d2 = d2 + D2p;
+            }
+            if (act && z == z2p) {
+                s2 = s2 - D1p * cf; s1 = s1 - D2p * cf; d2 = d2 + D1p; d1 = d1 + D2p;
+            }
+            const int nl = zep - zp, ia = z - zp, ib = z - (z2p + 1);
+            const bool va = act && ia >= 0 && ia < nl, vb = act && ib >= 0 && ib < nl;
+            float m1a = 0.0f, m2a = 0.0f, m1b = 0.0f, m2b = 0.0f;
+            if (va) { m1a = D1[yp + ia] - D1[yp + ia + 1]; m2a = D2[yp + ia] - D2[yp + ia + 1]; }
+            if (vb) { m1b = D1[yp + ib] - D1[yp + ib + 1]; m2b = D2[yp + ib] - D2[yp + ib + 1]; }
+            const int dl = z2p + 1 - zp;    // ia - ib, the same for every plane
+            // s1: +m1a (statement 2, iteration ia), +m2b (7, ib);  d1: -m1a (4, ia), -m2b (10, ib)
+            if (dl <= 0) {
+                if (va) { s1 = s1 + m1a * cf; d1 = d1 - m1a; }
+                if (vb) { s1 = s1 + m2b * cf; d1 = d1 - m2b; }
+            } else {
+                if (vb) { s1 = s1 + m2b * cf; d1 = d1 - m2b; }
+                if (va) { s1 = s1 + m1a * cf; d1 = d1 - m1a; }
+            }
+            // s2: +m1b (statement 3, ib), +m2a (8, ia);  d2: -m1b (5, ib), -m2a (9, ia)
+            if (dl < 0) {
+                if (va) { s2 = s2 + m2a * cf; d2 = d2 - m2a; }
+                if (vb) { s2 = s2 + m1b * cf; d2 = d2 - m1b; }
+            } else {
+                if (vb) { s2 = s2 + m1b * cf; d2 = d2 - m1b; }
+                if (va) { s2 = s2 + m2a * cf; d2 = d2 - m2a; }
+            }
+            const int ye = yp + nl, z2e = z2p + 1 + nl;
+            const float D1e = D1[ye], D2e = D2[ye];
+            if (act && z == zep) {
+                s1 = s1 + D1e * cf; s2 = s2 + D2e * cf; d1 = d1 - D1e; d2 = d2 - D2e;
+            }
+            if (act && z == z2e) {
+                s2 = s2 + D1e * cf; s1 = s1 + D2e * cf; d2 = d2 - D1e; d1 = d1 - D2e;
+            }
+        }
+        s1 = s1 + d1;
+        s2 = s2 + d2;
+        if (act) {
+            vox(y, false)[z] = s1;
+            vox(y, true)[z] = s2;
         }
     }
 }
@@ -937,6 +1136,15 @@ hipError_t launch_backproject(const DevGeom &g, const PairBatch &b, const void *
 }
 
 static dim3 rot_grid(const DevGeom &g, int nframes = 1) { return dim3(g.N * ((g.N + 3) / 4), nframes); }
+
+hipError_t launch_backproject_incremental(const DevGeom &g, const PairBatch &b, const float *src, int src_global,
+                                          int apply_rescale, float *bufA, float *bufB, hipStream_t s) {
+    if (!g.inc_ok) return hipErrorInvalidValue;   // tables failed the host's window checks
+    dim3 grid(g.c, 4, b.npairs * b.nframes), block(256);
+    hipLaunchKernelGGL(bp_inc_kernel, grid, block, sizeof(float) * g.inc_lds, s, g, b, src, src_global, apply_rescale,
+                       g.pseg, bufA, bufB);
+    return hipGetLastError();
+}
 
 hipError_t launch_rotate_update(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB,
                                 float *image, const float *normfac, int kahan, hipStream_t s) {

@@ -245,6 +245,13 @@ class Osem3D(Layout):
         _lib.call("hrrt_forward_proj3d", self.ctx, _p(image), _p(prj), int(view), _stream())
         return prj
 
+    def back_proj3d_incremental(self, prj, ima, view: int):
+        _lib.call("hrrt_back_proj3d_incremental", self.ctx, _p(prj), _p(ima), int(view), _stream())
+        return ima
+
+    def set_backproject_algorithm(self, incremental: bool):
+        _lib.call("hrrt_set_backproject_algorithm", self.ctx, int(bool(incremental)))
+
     def back_proj3d(self, prj, ima, view: int):
         _lib.call("hrrt_back_proj3d", self.ctx, _p(prj), _p(ima), int(view), _stream())
         return ima

@@ -226,6 +226,17 @@ int hrrt_forward_proj3d(hrrt_ctx *ctx, const float *d_image, float *d_prj, int v
 /* back_proj3d2 (PJ:1457): rescales d_prj in place (as the reference does),
  * back-projects both views of the pair and adds the rotated result into d_ima. */
 int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream);
+/* back_proj3d_thread1 (PJ:1622-1699, back_proj3d_view_n1thread PJ:4162-4668): the
+ * incremental back projector, the reference's default for most view pairs (the
+ * *2 variant serves the remainder, S3:4198-4218).  Same arguments and effect as
+ * hrrt_back_proj3d; the two differ by float rounding (about 1e-6 relative).
+ * HRRT_ERR_UNSUPPORTED when a breakpoint of the geometry moves the plane range
+ * by other than one plane (none does for the HRRT models). */
+int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream);
+/* Back projector used by the FP32 subset updates (hrrt_osem_subset, *_frames):
+ * 0 = direct (back_proj3d2), 1 = incremental (back_proj3d_thread1).  The
+ * precision form always uses the direct variant with compensated sums. */
+int hrrt_set_backproject_algorithm(hrrt_ctx *ctx, int incremental);
 /* update_estimate_w3 (S3:2474): est = prompt==0 ? 0 : (est+add > eps3 ? prompt/(est+add) : 1),
  * over the [2S][N] rows of one view pair. */
 int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float *d_add, const int16_t *d_prompt, void *stream);
