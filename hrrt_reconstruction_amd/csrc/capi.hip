@@ -384,83 +384,99 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
             }
         }
 
-    // incremental back projector (back_proj3d_view_n1thread PJ:4162-4668): one lane's column in
-    // LDS (segment 0, then segments 2k and 2k-1 each with a zero guard plane on both sides,
-    // PJ:1689-1692), the per-group difference factors, the first-row lookups (rows c-1, c+1)
-    // and the breakpoint schedule shared by every column (rows c-2 down to 1)
-    std::vector<int2> ibase(G.K + 1, make_int2(0, 0));
-    std::vector<int> isrc, iconv;
+    // incremental back projector (back_proj3d_view_n1thread PJ:4162-4668).  LDS holds one lane's
+    // column: segment 0 (Z floats, padded to an even count), then per group k a window of
+    // float2 {segment 2k, segment 2k-1} for rows yr = yb-3 .. yt+1, where yb-1 and yt+1 are the
+    // reference's zero guard planes (PJ:1689-1692) and yb-3, yb-2 two more zero cells.  The
+    // window is turned in place into the reference's differences P'(yr) = (p(yr+1)-p(yr))*tan
+    // (PJ:4313-4324), then into the values the breakpoint terms read: M(yr) = P'(yr)-P'(yr+1)
+    // for yr in [yb-1, yt-1] (the middle terms PJ:4381-4394), M(yb-2) = -P'(yb-1) (the two
+    // terms before the loop, PJ:4369-4380) and M(yt) = P'(yt) (the two after it, PJ:4395-4406),
+    // with zeros at yb-3 and yt+1: a breakpoint then adds, to plane z, the terms read at rows
+    // z + const clamped into the window, a clamped read giving exactly +0.
+    int inc_ok = Z <= 256 && N <= 256 && G.K <= 32;
+    std::vector<int> isrc;
+    std::vector<int2> iconv;
+    std::vector<int> gbase(G.K + 1, 0);
     for (int z = 0; z < Z; z++) {
         isrc.push_back(G.seg_data[0] + z);
-        iconv.push_back(0);
+        iconv.push_back(make_int2(0, 0));
+    }
+    if (Z & 1) {
+        isrc.push_back(-1);
+        iconv.push_back(make_int2(0, 0));
     }
     for (int k = 1; k <= G.K; k++) {
-        int bases[2];
-        for (int e = 0; e < 2; e++) {
-            const int sg = e == 0 ? 2 * k : 2 * k - 1, yb = G.yr_bottom[sg], yt = G.yr_top[sg];
-            bases[e] = (int)isrc.size() - (yb - 1);
-            for (int yr = yb - 1; yr <= yt + 1; yr++) {
+        const int th = 2 * k, yb = G.yr_bottom[th], yt = G.yr_top[th];
+        if (yb != G.yr_bottom[th - 1] || yt != G.yr_top[th - 1] || yt < yb) inc_ok = 0;
+        const float tn = -G.sin_theta[th] / G.cos_theta[th];
+        gbase[k] = (int)isrc.size() / 2 - (yb - 3);                  // float2 index of row 0
+        for (int yr = yb - 3; yr <= yt + 1; yr++)
+            for (int e = 0; e < 2; e++) {
+                const int sg = e == 0 ? th : th - 1;
                 isrc.push_back(yr >= yb && yr <= yt ? G.seg_data[sg] + (yr - yb) : -1);
-                iconv.push_back(yr <= yt ? k : 0);
+                const int second = yr == yb - 2 ? 2 : (yr >= yb - 1 && yr <= yt - 1 ? 1 : 0);
+                iconv.push_back(make_int2(yr >= yb - 1 && yr <= yt ? bits(tn) : 0, second));
             }
-        }
-        ibase[k] = make_int2(bases[0], bases[1]);
     }
     const int inc_lds = (int)isrc.size();
-    isrc.push_back(-1);
-    iconv.push_back(0);
-    std::vector<float> itn(G.K + 1, 0.f);
-    std::vector<int> ifirst((size_t)(G.K + 1) * 12, 0);
-    for (int k = 1; k <= G.K; k++) {
-        const int th = 2 * k;
-        itn[k] = -G.sin_theta[th] / G.cos_theta[th];
-        const size_t l1 = (size_t)(c - 1) * GM + k - 1, l2 = (size_t)(c + 1) * GM + k - 1;
-        const hrrt::Geometry::Blut &bc = G.blut[l2];
-        int *f = &ifirst[(size_t)k * 12];
-        f[0] = G.bl_zsa[l1]; f[1] = G.bl_zea[l1]; f[2] = G.bl_yra[l1]; f[3] = bits(G.bl_frac[l1]);
-        f[4] = G.bl_zsa[l2]; f[5] = G.bl_yra[l2]; f[6] = bits(G.bl_frac[l2]);
-        f[7] = bc.z_start; f[8] = bc.z_end; f[9] = bc.yr_start;
-    }
-    std::vector<int4> isched;
-    std::vector<int2> irow(c + 1, make_int2(0, 0));
-    // every difference-column read must stay inside its segment's window [yb-1, yt+1], and
-    // one thread per plane needs Z <= 256
-    int inc_ok = Z <= 256;
     auto in_window = [&](int k, int first, int count) {      // rows first .. first+count of group k
         if (count < 0) return true;
-        for (int sg = 2 * k - 1; sg <= 2 * k; sg++)
-            if (first < G.yr_bottom[sg] - 1 || first + count > G.yr_top[sg] + 1) return false;
-        return true;
+        return first >= G.yr_bottom[2 * k] - 1 && first + count <= G.yr_top[2 * k] + 1;
     };
+    // per group k (entry k-1): {float2 index of row 0 of the window, yb, yt, 0}, row c-1 lookup
+    // {zsa, count, yra, bits(frac)}, row c+1 lookup {zsa, yra, bits(frac), 0}, row c+1 breakpoint
+    // {z_start, z_end, yr_start, 0}; a range with no planes points at yb so that its (unused)
+    // reads stay in the window
+    std::vector<int4> igroup((size_t)std::max(G.K, 1) * 4, make_int4(0, 0, 0, 0));
     for (int k = 1; k <= G.K; k++) {
-        const int *f = &ifirst[(size_t)k * 12];
-        const int n = f[1] - f[0] + 1;
-        if (G.yr_bottom[2 * k] != G.yr_bottom[2 * k - 1] || G.yr_top[2 * k] != G.yr_top[2 * k - 1]) inc_ok = 0;
-        if (n > 0 && (!in_window(k, f[2], n) || !in_window(k, f[5], n))) inc_ok = 0;
-        if (!in_window(k, f[9], f[8] - f[7])) inc_ok = 0;
+        const int th = 2 * k, yb = G.yr_bottom[th], yt = G.yr_top[th];
+        const size_t l1 = (size_t)(c - 1) * GM + k - 1, l2 = (size_t)(c + 1) * GM + k - 1;
+        const hrrt::Geometry::Blut &bc = G.blut[l2];
+        const int n = G.bl_zea[l1] - G.bl_zsa[l1] + 1;
+        int yra1 = G.bl_yra[l1], yra2 = G.bl_yra[l2], bcyr = bc.yr_start;
+        if (n <= 0) yra1 = yra2 = yb;
+        if (bc.z_end < bc.z_start) bcyr = yb;
+        if (n > 0 && (!in_window(k, yra1, n) || !in_window(k, yra2, n))) inc_ok = 0;
+        if (!in_window(k, bcyr, std::max(bc.z_end - bc.z_start, 0))) inc_ok = 0;
+        int4 *e = &igroup[(size_t)(k - 1) * 4];
+        e[0] = make_int4(gbase[k], yb, yt, 0);
+        e[1] = make_int4(G.bl_zsa[l1], n, yra1, bits(G.bl_frac[l1]));
+        e[2] = make_int4(G.bl_zsa[l2], yra2, bits(G.bl_frac[l2]), 0);
+        e[3] = make_int4(bc.z_start, bc.z_end, bcyr, 0);
     }
+    // breakpoints of rows c-2 down to 1, groups ascending: {bits(coef1), byte offsets (from 8z) of
+    // the a-term row yr_start_pre + (z - z_start_pre) and of the b-term row yr_start_pre +
+    // (z - z_start_pre(N-y) - 1), window bounds lo | hi << 16 in bytes}.  Handled: the range moved
+    // down by one plane, it spans the segment's rows [yb-1, yt], and the b-term's iteration of a
+    // plane is never after its a-term's (z_start_pre(N-y) + 1 > z_start_pre), so every plane takes
+    // its b-term first (PJ:4381-4394)
+    std::vector<int4> isched;
+    std::vector<int> inrow(c + 1, 0);
     for (int y = c - 2; y >= 1; y--) {
-        const int off = (int)isched.size() / 2;
         for (int k = 1; k <= G.K; k++) {
             const size_t li = (size_t)y * GM + k - 1;
             if (G.blut_flag[li] == 0) continue;
             const hrrt::Geometry::Blut &ba = G.blut[li], &bb = G.blut[(size_t)(N - y) * GM + k - 1];
-            if (ba.z_start - ba.z_start_pre != -1) inc_ok = 0;   // the range moved by other than one plane
-            if (ba.z_end_pre < ba.z_start_pre || !in_window(k, ba.yr_start_pre, ba.z_end_pre - ba.z_start_pre))
-                inc_ok = 0;
-            const int th = 2 * k;
+            const int th = 2 * k, yb = G.yr_bottom[th], yt = G.yr_top[th];
+            const int zp = ba.z_start_pre, z2p = bb.z_start_pre, nl = ba.z_end_pre - ba.z_start_pre;
+            if (ba.z_start - zp != -1) inc_ok = 0;                     // the range moved by other than one plane
+            if (ba.yr_start_pre != yb - 1 || ba.yr_start_pre + nl != yt) inc_ok = 0;
+            if (z2p + 1 - zp <= 0) inc_ok = 0;
             float tn = -G.sin_theta[th] / G.cos_theta[th];
             tn = 1 - tn * (float)ba.num - ba.coef_pre;                    // PJ:4371-4373
             const float coef1 = -tn * G.cos_theta[th] / G.sin_theta[th];
-            isched.push_back(make_int4(k, bits(coef1), ba.z_start_pre, bb.z_start_pre));
-            isched.push_back(make_int4(ba.yr_start_pre, ba.z_start, ba.z_end_pre, 0));
+            const int offa = 8 * (gbase[k] + ba.yr_start_pre - zp), offb = 8 * (gbase[k] + ba.yr_start_pre - z2p - 1);
+            const int lo8 = 8 * (gbase[k] + yb - 3), hi8 = 8 * (gbase[k] + yt + 1);
+            isched.push_back(make_int4(bits(coef1), offa, offb, lo8 | (hi8 << 16)));
+            inrow[y]++;
         }
-        irow[y] = make_int2(off, (int)isched.size() / 2 - off);
     }
-    if (isched.empty()) isched.push_back(make_int4(0, 0, 0, 0));
+    isched.push_back(make_int4(0, 0, 0, 0));                           // read ahead of the last one
     d.inc_ok = inc_ok;
     d.inc_lds = inc_lds;
-    if ((size_t)inc_lds * sizeof(float) > 64 * 1024) d.inc_ok = 0;
+    d.inc_shm = (int)(sizeof(float) * (inc_lds + 2));                // + the two floats the last conversion reads
+    if (d.inc_shm > 64 * 1024) d.inc_ok = 0;
 
     int r = 0;
     if ((r = ctx->upload(G.cyl_lo, d.cyl_lo)) || (r = ctx->upload(G.cyl_hi, d.cyl_hi)) ||
@@ -472,9 +488,8 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         (r = ctx->upload(rot, d.rot)) || (r = ctx->upload(fplanes, d.fp_planes)) || (r = ctx->upload(pseg, d.pseg)) ||
         (r = ctx->upload(plane_seg, d.plane_seg)) || (r = ctx->upload(bp_src, d.bp_src)) ||
         (r = ctx->upload(blut2, d.bp_lut2)) || (r = ctx->upload(isrc, d.inc_src)) ||
-        (r = ctx->upload(iconv, d.inc_conv)) || (r = ctx->upload(itn, d.inc_tn)) ||
-        (r = ctx->upload(ibase, d.inc_base)) || (r = ctx->upload(ifirst, d.inc_first)) ||
-        (r = ctx->upload(isched, d.inc_sched)) || (r = ctx->upload(irow, d.inc_row))) {
+        (r = ctx->upload(iconv, d.inc_conv)) || (r = ctx->upload(igroup, d.inc_group)) ||
+        (r = ctx->upload(isched, d.inc_sched)) || (r = ctx->upload(inrow, d.inc_nrow))) {
         delete ctx;
         return r;
     }

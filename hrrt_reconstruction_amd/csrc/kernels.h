@@ -35,14 +35,13 @@ struct DevGeom {
     int8_t *nzmask;                      // [views/2][N/2+1], 0 = masked column
     // incremental back projector (back_proj3d_view_n1thread PJ:4162-4668)
     int inc_ok;                          // 1 when every breakpoint of the geometry is of the kind handled
-    int inc_lds;                         // LDS floats of one lane's column (segments with guard planes)
-    const int *inc_src;                  // [inc_lds] LDS entry -> pair-column plane (-1 = zero)
-    const int *inc_conv;                 // [inc_lds] group k whose tan converts the entry to a difference (0 = none)
-    const float *inc_tn;                 // [K+1] -sin(theta)/cos(theta) of segment 2k
-    const int2 *inc_base;                // [K+1] LDS index of plane 0 of segments 2k, 2k-1
-    const int *inc_first;                // [K+1][12] first-row lookups (rows c-1, c+1) and breakpoint of row c+1
-    const int4 *inc_sched;               // [2*n] breakpoints, rows c-2 down to 1, groups ascending
-    const int2 *inc_row;                 // [c+1] first entry and count of row y
+    int inc_lds;                         // LDS floats of one lane's column (segment 0, then one window per group)
+    int inc_shm;                         // LDS bytes of the incremental back projector's workgroup
+    const int *inc_src;                  // [inc_lds] LDS float -> pair-column plane (-1 = zero)
+    const int2 *inc_conv;                // [inc_lds] {bits(-tan) of the difference step (0 = none), second step}
+    const int4 *inc_group;               // [K][4] per group: window, first-row lookups, row c+1 breakpoint
+    const int4 *inc_sched;               // [n+1] breakpoints of rows c-2 down to 1, groups ascending
+    const int *inc_nrow;                 // [c+1] breakpoints of row y
     int fp_maxt, fp_nsplit;              // forward-projector launch shape (tasks/thread, workgroups/(pair,x))
     int bp_waves;                        // back-projector waves per workgroup
 };

@@ -485,35 +485,36 @@ __global__ void __launch_bounds__(1024) bp_kernel(const DevGeom g, const PairBat
 // increments are corrected.  One workgroup per (column x <= N/2, lane, view
 // pair); lane l is bin x or N-x (l & 1) of view v or v+90 (l >> 1); a thread
 // owns one plane and applies, to its own accumulators, exactly the terms the
-// reference's loops apply to that plane, in the reference's order (the order
-// of two terms of one breakpoint depends only on the breakpoint, so it is
-// wave-uniform).  Lanes 0/1 write the rotated-frame half A, lanes 2/3 the half
-// B, so each voxel of A and of B has one writer and R = A + B as for the
-// direct variant.
+// reference's loops apply to that plane, in the reference's order.  Lanes 0/1
+// write the rotated-frame half A, lanes 2/3 the half B, so each voxel of A and
+// of B has one writer and R = A + B as for the direct variant.
+//
+// LDS: segment 0, then per group a window of float2 {segment 2k, 2k-1} by row
+// (layout and the two in-place conversions are described with the host
+// tables).  A breakpoint is then, for every plane, two clamped float2 reads
+// and eight fixed operations (b-term first, then a-term, each one s += m*coef,
+// d -= m for both sums); the reads that fall outside the breakpoint's rows hit
+// the window's zero cells and add +0.  Breakpoint records come through the
+// scalar cache one ahead of use; per-group constants sit in VGPRs (lane k-1).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float rlf(int v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(v, lane)); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
 __global__ void __launch_bounds__(256) bp_inc_kernel(const DevGeom g, const PairBatch b, const float *__restrict__ src,
                                                      int src_global, int apply_rescale,
                                                      const int16_t *__restrict__ pseg, float *__restrict__ bufA,
                                                      float *__restrict__ bufB) {
     extern __shared__ float P[];
+    const float2 *W = reinterpret_cast<const float2 *>(P);
     const int x = blockIdx.x + 1, l = blockIdx.y, lp = blockIdx.z % b.npairs, frame = blockIdx.z / b.npairs;
     const int N = g.N, c = g.c, Z = g.Z, K = g.K;
     if (x == c && (l & 1)) return;          // at x == N/2 lanes 1/3 duplicate lanes 0/2
     const int lo = g.cyl_lo[x], hi = g.cyl_hi[x];
     if (lo == hi) return;
     const int view = b.view0 + b.view_stride * lp;
-    src += frame * b.ratio_stride;
     const int h = l >> 1, col = (l & 1) ? N - x : x;
-    const int gp = src_global ? view : lp;
-    const float *cp = src + ((size_t)(gp * 2 + h) * N + col) * g.spp;
-    for (int i = threadIdx.x; i < g.inc_lds; i += blockDim.x) {   // transpose PJ:1654-1693
-        const int si = g.inc_src[i];
-        float v = 0.0f;
-        if (si >= 0) v = apply_rescale ? cp[si] * g.bp_rescale[pseg[si]] : cp[si];
-        P[i] = v;
-    }
-    __syncthreads();
-    const int z = threadIdx.x;
+    const int z = threadIdx.x, lane = threadIdx.x & 63;
     const bool act = z < Z;
     float *dst = (l < 2 ? bufA : bufB) + frame * b.buf_stride + (size_t)lp * N * N * g.Z4;
     // s1 of row y goes to A(col, y) / B(N-y, col), s2 to A(col, N-y) / B(y, col)
@@ -533,133 +534,136 @@ __global__ void __launch_bounds__(256) bp_inc_kernel(const DevGeom g, const Pair
         }
         return;
     }
-    // centre voxels PJ:4265-4285
-    if (act) {
-        float sv = P[z];
-        for (int k = 1; k <= K; k++) {
-            const int2 bs = g.inc_base[k];
-            if (z >= g.yb[2 * k] && z <= g.yt[2 * k]) sv = sv + (P[bs.x + z] + P[bs.y + z]);
+    src += frame * b.ratio_stride;
+    const int gp = src_global ? view : lp;
+    const float *cp = src + ((size_t)(gp * 2 + h) * N + col) * g.spp;
+    for (int base = 0; base < g.inc_lds; base += 4 * (int)blockDim.x) {   // transpose PJ:1654-1693
+        int si[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i = base + threadIdx.x + j * blockDim.x;
+            si[j] = i < g.inc_lds ? g.inc_src[i] : -1;
         }
-        vox(c, false)[z] = sv;
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            v[j] = 0.0f;
+            if (si[j] >= 0) v[j] = apply_rescale ? cp[si[j]] * g.bp_rescale[pseg[si[j]]] : cp[si[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int i = base + threadIdx.x + j * blockDim.x;
+            if (i < g.inc_lds) P[i] = v[j];
+        }
+    }
+    // per-group constants of group lane+1
+    int4 e0 = make_int4(0, 0, 0, 0), e1 = e0, e2 = e0, e3 = e0;
+    if (lane < K) {
+        const int4 *q = g.inc_group + 4 * lane;
+        e0 = q[0]; e1 = q[1]; e2 = q[2]; e3 = q[3];
+    }
+    __syncthreads();
+    const int zc = act ? z : 0;
+    // centre voxels PJ:4265-4285
+    {
+        float sv = P[zc];
+        for (int k = 0; k < K; k++) {
+            const int gb = rl(e0.x, k), yb = rl(e0.y, k), yt = rl(e0.z, k);
+            const float2 w = W[gb + clampi(z, yb, yt)];
+            const float t = sv + (w.x + w.y);
+            sv = (z >= yb && z <= yt) ? t : sv;
+        }
+        if (act) vox(c, false)[z] = sv;
     }
     // first row y = c-1 by direct interpolation PJ:4287-4312
-    float s1 = act ? P[z] : 0.0f, s2 = s1, d1 = 0.0f, d2 = 0.0f;
-    for (int k = 1; k <= K; k++) {
-        const int *f = g.inc_first + k * 12;
-        const int2 bs = g.inc_base[k];
-        const int n = f[1] - f[0] + 1;
-        const float c1 = __int_as_float(f[3]), c2 = __int_as_float(f[6]);
-        const int ia = z - f[0], ib = z - f[4];
-        const bool va = act && ia >= 0 && ia < n, vb = act && ib >= 0 && ib < n;
-        float a1 = 0.0f, a2 = 0.0f, b1 = 0.0f, b2 = 0.0f;
-        if (va) {
-            const int yr = f[2] + ia;
-            float p = P[bs.x + yr], q = P[bs.x + yr + 1];
-            a1 = p + c1 * (q - p);
-            p = P[bs.y + yr]; q = P[bs.y + yr + 1];
-            a2 = p + c1 * (q - p);
-        }
-        if (vb) {
-            const int yr = f[5] + ib;
-            float p = P[bs.y + yr], q = P[bs.y + yr + 1];
-            b1 = p + c2 * (q - p);
-            p = P[bs.x + yr]; q = P[bs.x + yr + 1];
-            b2 = p + c2 * (q - p);
-        }
-        if (f[4] <= f[0]) {          // iteration ia <= ib: the row-y term first
-            if (va) { s1 = s1 + a1; s2 = s2 + a2; }
-            if (vb) { s1 = s1 + b1; s2 = s2 + b2; }
+    float s1 = P[zc], s2 = s1, d1 = 0.0f, d2 = 0.0f;
+    for (int k = 0; k < K; k++) {
+        const int gb = rl(e0.x, k);
+        const int za = rl(e1.x, k), n = rl(e1.y, k), ya = rl(e1.z, k), zb = rl(e2.x, k), yb2 = rl(e2.y, k);
+        const float c1 = rlf(e1.w, k), c2 = rlf(e2.z, k);
+        const int ia = z - za, ib = z - zb, nm = n > 1 ? n - 1 : 0;
+        const bool va = ia >= 0 && ia < n, vb = ib >= 0 && ib < n;
+        const int ra = gb + ya + clampi(ia, 0, nm), rb = gb + yb2 + clampi(ib, 0, nm);
+        const float2 pa = W[ra], qa = W[ra + 1], pb = W[rb], qb = W[rb + 1];
+        const float a1 = pa.x + c1 * (qa.x - pa.x), a2 = pa.y + c1 * (qa.y - pa.y);
+        const float b1 = pb.y + c2 * (qb.y - pb.y), b2 = pb.x + c2 * (qb.x - pb.x);
+        if (zb <= za) {              // iteration ia <= ib: the row-y term first
+            s1 = va ? s1 + a1 : s1; s2 = va ? s2 + a2 : s2;
+            s1 = vb ? s1 + b1 : s1; s2 = vb ? s2 + b2 : s2;
         } else {
-            if (vb) { s1 = s1 + b1; s2 = s2 + b2; }
-            if (va) { s1 = s1 + a1; s2 = s2 + a2; }
+            s1 = vb ? s1 + b1 : s1; s2 = vb ? s2 + b2 : s2;
+            s1 = va ? s1 + a1 : s1; s2 = va ? s2 + a2 : s2;
         }
     }
     __syncthreads();
-    // the columns become scaled differences, in place PJ:4313-4330
-    for (int base = 0; base < g.inc_lds; base += 4 * (int)blockDim.x) {
-        float nv[4];
+    // in place, twice: the reference's scaled differences P' (PJ:4313-4324), then the values the
+    // breakpoint terms read
+    for (int pass = 0; pass < 2; pass++) {
+        for (int base = 0; base < g.inc_lds; base += 4 * (int)blockDim.x) {
+            float nv[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int i = base + threadIdx.x + j * blockDim.x;
-            nv[j] = 0.0f;
-            if (i < g.inc_lds) {
-                const int k = g.inc_conv[i];
-                nv[j] = k ? (P[i + 1] - P[i]) * g.inc_tn[k] : P[i];
+            for (int j = 0; j < 4; j++) {
+                const int i = base + threadIdx.x + j * blockDim.x;
+                nv[j] = 0.0f;
+                if (i < g.inc_lds) {
+                    const int2 cv = g.inc_conv[i];
+                    const float p = P[i];
+                    if (pass == 0) nv[j] = cv.x ? (P[i + 2] - p) * __int_as_float(cv.x) : p;
+                    else nv[j] = cv.y == 1 ? p - P[i + 2] : (cv.y == 2 ? -P[i + 2] : p);
+                }
             }
-        }
-        __syncthreads();
+            __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int i = base + threadIdx.x + j * blockDim.x;
-            if (i < g.inc_lds) P[i] = nv[j];
+            for (int j = 0; j < 4; j++) {
+                const int i = base + threadIdx.x + j * blockDim.x;
+                if (i < g.inc_lds) P[i] = nv[j];
+            }
+            __syncthreads();
+        }
+        if (pass == 1) break;
+        for (int k = 0; k < K; k++) {                                // PJ:4325-4335
+            const int gb = rl(e0.x, k), yb = rl(e0.y, k), yt = rl(e0.z, k);
+            const int zs = rl(e3.x, k), ze = rl(e3.y, k), yr0 = rl(e3.z, k);
+            const bool vd = z >= yb - 1 && z <= yt;
+            const float2 w = W[gb + clampi(z, yb - 1, yt)];
+            const float t1 = d1 + w.x, t2 = d2 + w.y;
+            d1 = vd ? t1 : d1; d2 = vd ? t2 : d2;
+            const int j = z - zs;
+            const bool vj = j >= 0 && j <= ze - zs;
+            const float2 u = W[gb + yr0 + clampi(j, 0, ze > zs ? ze - zs : 0)];
+            const float u1 = d1 - u.y, u2 = d2 - u.x;
+            d1 = vj ? u1 : d1; d2 = vj ? u2 : d2;
         }
         __syncthreads();
-    }
-    for (int k = 1; k <= K; k++) {                                   // PJ:4325-4335
-        const int *f = g.inc_first + k * 12;
-        const int2 bs = g.inc_base[k];
-        if (act && z >= g.yb[2 * k] - 1 && z <= g.yt[2 * k]) {
-            d1 = d1 + P[bs.x + z];
-            d2 = d2 + P[bs.y + z];
-        }
-        const int j = z - f[7];
-        if (act && j >= 0 && j <= f[8] - f[7]) {
-            const int yr = f[9] + j;
-            d1 = d1 - P[bs.y + yr];
-            d2 = d2 - P[bs.x + yr];
-        }
     }
     if (act) {
         vox(c - 1, false)[z] = s1;
         vox(c - 1, true)[z] = s2;
     }
-    // rows c-2 .. ystart PJ:4337-4562: breakpoints (the range moved down by one plane), then
-    // the running sums advance by their increments
+    // rows c-2 .. ystart PJ:4337-4562: breakpoints, then the running sums advance by their
+    // increments
+    const int z8 = 8 * z;
+    const char *Wb = reinterpret_cast<const char *>(P);
+    const int4 *__restrict__ T = g.inc_sched;
+    int e = 0;
+    int4 cur = T[0];
     for (int y = c - 2; y >= lo; y--) {
-        const int2 rr = g.inc_row[y];
-        for (int e = rr.x; e < rr.x + rr.y; e++) {
-            const int4 q0 = g.inc_sched[2 * e], q1 = g.inc_sched[2 * e + 1];
-            const int k = q0.x, zp = q0.z, z2p = q0.w, yp = q1.x, z3 = q1.y, zep = q1.z;
-            const float cf = __int_as_float(q0.y);
-            const int2 bs = g.inc_base[k];
-            const float *D1 = P + bs.x, *D2 = P + bs.y;
-            const float D1p = D1[yp], D2p = D2[yp];
-            if (act && z == z3) {
-                s1 = s1 - D1p * cf; s2 = s2 - D2p * cf; d1 = d1 + D1p; d2 = d2 + D2p;
-            }
-            if (act && z == z2p) {
-                s2 = s2 - D1p * cf; s1 = s1 - D2p * cf; d2 = d2 + D1p; d1 = d1 + D2p;
-            }
-            const int nl = zep - zp, ia = z - zp, ib = z - (z2p + 1);
-            const bool va = act && ia >= 0 && ia < nl, vb = act && ib >= 0 && ib < nl;
-            float m1a = 0.0f, m2a = 0.0f, m1b = 0.0f, m2b = 0.0f;
-            if (va) { m1a = D1[yp + ia] - D1[yp + ia + 1]; m2a = D2[yp + ia] - D2[yp + ia + 1]; }
-            if (vb) { m1b = D1[yp + ib] - D1[yp + ib + 1]; m2b = D2[yp + ib] - D2[yp + ib + 1]; }
-            const int dl = z2p + 1 - zp;    // ia - ib, the same for every plane
-            // s1: +m1a (statement 2, iteration ia), +m2b (7, ib);  d1: -m1a (4, ia), -m2b (10, ib)
-            if (dl <= 0) {
-                if (va) { s1 = s1 + m1a * cf; d1 = d1 - m1a; }
-                if (vb) { s1 = s1 + m2b * cf; d1 = d1 - m2b; }
-            } else {
-                if (vb) { s1 = s1 + m2b * cf; d1 = d1 - m2b; }
-                if (va) { s1 = s1 + m1a * cf; d1 = d1 - m1a; }
-            }
-            // s2: +m1b (statement 3, ib), +m2a (8, ia);  d2: -m1b (5, ib), -m2a (9, ia)
-            if (dl < 0) {
-                if (va) { s2 = s2 + m2a * cf; d2 = d2 - m2a; }
-                if (vb) { s2 = s2 + m1b * cf; d2 = d2 - m1b; }
-            } else {
-                if (vb) { s2 = s2 + m1b * cf; d2 = d2 - m1b; }
-                if (va) { s2 = s2 + m2a * cf; d2 = d2 - m2a; }
-            }
-            const int ye = yp + nl, z2e = z2p + 1 + nl;
-            const float D1e = D1[ye], D2e = D2[ye];
-            if (act && z == zep) {
-                s1 = s1 + D1e * cf; s2 = s2 + D2e * cf; d1 = d1 - D1e; d2 = d2 - D2e;
-            }
-            if (act && z == z2e) {
-                s2 = s2 + D1e * cf; s1 = s1 + D2e * cf; d2 = d2 - D1e; d1 = d1 - D2e;
-            }
+        const int n = g.inc_nrow[y];
+        for (int j = 0; j < n; j++) {
+            const int4 nxt = T[e + 1];
+            const float cf = __int_as_float(cur.x);
+            const int lo8 = cur.w & 0xffff, hi8 = cur.w >> 16;
+            const int ra = clampi(z8 + cur.y, lo8, hi8), rb = clampi(z8 + cur.z, lo8, hi8);
+            const float2 ma = *reinterpret_cast<const float2 *>(Wb + ra);
+            const float2 mb = *reinterpret_cast<const float2 *>(Wb + rb);
+            // b-term (PJ:4386-4391 for rows N-y): s1 += m2*coef, s2 += m1*coef, d1 -= m2, d2 -= m1
+            s1 = s1 + mb.y * cf; d1 = d1 - mb.y;
+            s2 = s2 + mb.x * cf; d2 = d2 - mb.x;
+            // a-term (PJ:4382-4385): s1 += m1*coef, s2 += m2*coef, d1 -= m1, d2 -= m2
+            s1 = s1 + ma.x * cf; d1 = d1 - ma.x;
+            s2 = s2 + ma.y * cf; d2 = d2 - ma.y;
+            cur = nxt;
+            e++;
         }
         s1 = s1 + d1;
         s2 = s2 + d2;
@@ -1141,7 +1145,7 @@ hipError_t launch_backproject_incremental(const DevGeom &g, const PairBatch &b, 
                                           int apply_rescale, float *bufA, float *bufB, hipStream_t s) {
     if (!g.inc_ok) return hipErrorInvalidValue;   // tables failed the host's window checks
     dim3 grid(g.c, 4, b.npairs * b.nframes), block(256);
-    hipLaunchKernelGGL(bp_inc_kernel, grid, block, sizeof(float) * g.inc_lds, s, g, b, src, src_global, apply_rescale,
+    hipLaunchKernelGGL(bp_inc_kernel, grid, block, g.inc_shm, s, g, b, src, src_global, apply_rescale,
                        g.pseg, bufA, bufB);
     return hipGetLastError();
 }
