@@ -279,13 +279,13 @@ def main():
     image = g.init_image()
     image0 = image.clone()
     order = [int(s) for s in g.order]
-    shape, tune, bp_waves = None, {}, None
+    shape, tune, bp_choice = None, {}, None
     if not args.no_tune:
         from hrrt_reconstruction_amd.autotune import tune_backproject, tune_forward
         shape, tune = tune_forward(g, image, normfac, prompt, add)
         log("autotune forward " + ", ".join(f"{k}: {v:.3f} ms" for k, v in tune.items()) + f" -> {shape}")
-        bp_waves, tune_bp = tune_backproject(g, image, normfac, prompt, add)
-        log("autotune back " + ", ".join(f"{k}: {v:.3f} ms" for k, v in tune_bp.items()) + f" -> {bp_waves}")
+        bp_choice, tune_bp = tune_backproject(g, image, normfac, prompt, add)
+        log("autotune back " + ", ".join(f"{k}: {v:.3f} ms" for k, v in tune_bp.items()) + f" -> {bp_choice}")
     for i in range(args.warmup):
         g.osem_subset(image, normfac, prompt, add, order[i % g.subsets])
     torch.cuda.synchronize()
@@ -338,7 +338,8 @@ def main():
                                "randoms 10%, scatter 15%, 16 subsets (9 view pairs per subset)",
                    "subsets": g.subsets, "view_pairs_per_subset": npairs,
                    "stage_s": round(stage_s, 3), "normfac_s": round(normfac_s, 3),
-                   "fp_launch_shape": list(shape) if shape else None, "bp_waves_per_workgroup": bp_waves},
+                   "fp_launch_shape": list(shape) if shape else None, "back_projector": bp_choice,
+                   "backproject_schedule": g.backproject_schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic},

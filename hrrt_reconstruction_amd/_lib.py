@@ -74,7 +74,8 @@ SIGNATURES = {
     "hrrt_forward_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_back_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_back_proj3d_incremental": (_I, [_VP, _VP, _VP, _I, _VP]),
-    "hrrt_set_backproject_algorithm": (_I, [_VP, _I]),
+    "hrrt_set_backproject_schedule": (_I, [_VP, _I]),
+    "hrrt_get_backproject_schedule": (_I, [_VP, _VP]),
     "hrrt_update_estimate_w3": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "hrrt_cal_updateimage": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "hrrt_normfac_calculate": (_I, [_VP, _VP, _VP, _VP]),

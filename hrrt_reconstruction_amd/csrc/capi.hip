@@ -48,7 +48,7 @@ struct hrrt_ctx {
     uint8_t *gapmask = nullptr;      // [th][def_angles][def_elements]
     float *h_img = nullptr, *h_prj = nullptr;  // device staging for the host shims
     int device = 0;
-    int bp_incremental = 0;          // back projector of the FP32 subset path: 0 direct, 1 incremental
+    int bp_threads = 0;              // back-projection schedule (the reference's -T): 0 = direct for every pair
     // optional per-kernel timing of hrrt_osem_subset (HIP events on the launch stream)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -497,6 +497,7 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
     d.fp_maxt = 4;
     d.fp_nsplit = 0;
     d.bp_waves = 16;
+    ctx->bp_threads = d.inc_ok ? 1 : 0;                                 // every pair incremental when possible
     {
         std::vector<int8_t> nz((size_t)(G.views / 2) * (c + 1), 1);
         void *p = nullptr;
@@ -604,6 +605,33 @@ extern "C" int hrrt_stage_sinograms(hrrt_ctx *ctx, const int16_t *d_prompt_file,
     return HRRT_OK;
 }
 
+// Back projection of the pairs of a batch under the reference's thread schedule
+// (CalculateNormfac S3:3796-3825, CalculateOsem3d S3:4174-4218): with T threads the first
+// npairs - npairs % T pairs of a subset take the incremental projector (back_proj3d_thread1),
+// the rest the direct one (back_proj3d2); T = 0 takes the direct one for every pair.
+static hipError_t backproject_scheduled(const hrrt_ctx *ctx, const hrrt::PairBatch &b, const float *src,
+                                        int src_global, int apply_rescale, float *bufA, float *bufB, hipStream_t s) {
+    const hrrt::DevGeom &g = ctx->dg;
+    const int T = ctx->bp_threads;
+    const int nfull = T > 0 ? b.npairs - b.npairs % T : 0;
+    if (nfull > 0) {
+        hrrt::PairBatch bi = b;
+        bi.npairs = nfull;
+        hipError_t e = hrrt::launch_backproject_incremental(g, bi, src, src_global, apply_rescale, bufA, bufB, s);
+        if (e != hipSuccess) return e;
+    }
+    if (nfull < b.npairs) {
+        hrrt::PairBatch bd = b;
+        bd.view0 = b.view0 + b.view_stride * nfull;
+        bd.npairs = b.npairs - nfull;
+        const size_t vol = (size_t)g.N * g.N * g.Z4;
+        const float *sp = src_global ? src : src + (size_t)nfull * 2 * g.N * g.spp;
+        return hrrt::launch_backproject(g, bd, sp, 0, src_global, apply_rescale, 0, bufA + nfull * vol,
+                                        bufB + nfull * vol, s);
+    }
+    return hipSuccess;
+}
+
 extern "C" int hrrt_calculate_normfac(hrrt_ctx *ctx, const float *d_invnorm, float *d_normfac, void *stream) {
     if (!ctx || !d_invnorm || !d_normfac) return fail(HRRT_ERR_ARG, "null argument");
     int r;
@@ -611,7 +639,7 @@ extern "C" int hrrt_calculate_normfac(hrrt_ctx *ctx, const float *d_invnorm, flo
     const hrrt::Geometry &G = ctx->geo;
     for (int s = 0; s < G.p.subsets; s++) {
         hrrt::PairBatch b{s, G.p.subsets, ctx->npairs()};
-        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, d_invnorm, 0, 1, 1, 0, ctx->bufA, ctx->bufB, S(stream)));
+        HIP_TRY(backproject_scheduled(ctx, b, d_invnorm, 1, 1, ctx->bufA, ctx->bufB, S(stream)));
         HIP_TRY(hrrt::launch_rotate_sens(ctx->dg, b, ctx->bufA, ctx->bufB, d_normfac + (size_t)s * ctx->vol_floats(),
                                          0, S(stream)));
     }
@@ -644,10 +672,7 @@ extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_no
     io.add = d_add;
     HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, io, hrrt::FP_RATIO, S(stream)));
     ctx->mark(1, S(stream));
-    if (ctx->bp_incremental)
-        HIP_TRY(hrrt::launch_backproject_incremental(ctx->dg, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
-    else
-        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->sub_ratio, 0, 0, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    HIP_TRY(backproject_scheduled(ctx, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
     ctx->mark(2, S(stream));
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, 0, S(stream)));
     ctx->mark(-1, S(stream));
@@ -693,10 +718,10 @@ static int frames_subset_launch(hrrt_ctx *ctx, int nframes, float *d_images, con
         hrrt::forward_shape(ctx->dg, maxt, nsplit, threads, items);
         HIP_TRY(hrrt::launch_ll_reduce(ctx->ll_part, ctx->dg.c * nsplit * b.npairs, nframes, pr.loglik, s));
     }
-    if (ctx->bp_incremental && !pr.mixed)
-        HIP_TRY(hrrt::launch_backproject_incremental(ctx->dg, b, ctx->f_ratio, 0, 0, ctx->fA, ctx->fB, s));
+    if (pr.mixed)
+        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->f_ratio, 0, 0, 0, 1, ctx->fA, ctx->fB, s));
     else
-        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->f_ratio, 0, 0, 0, pr.mixed, ctx->fA, ctx->fB, s));
+        HIP_TRY(backproject_scheduled(ctx, b, ctx->f_ratio, 0, 0, ctx->fA, ctx->fB, s));
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->fA, ctx->fB, d_images, d_normfac_subset, pr.mixed, s));
     return HRRT_OK;
 }
@@ -927,12 +952,19 @@ extern "C" int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *
     return HRRT_OK;
 }
 
-extern "C" int hrrt_set_backproject_algorithm(hrrt_ctx *ctx, int incremental) {
+extern "C" int hrrt_set_backproject_schedule(hrrt_ctx *ctx, int nthreads) {
     if (!ctx) return fail(HRRT_ERR_ARG, "null context");
-    if (incremental && !ctx->dg.inc_ok)
+    if (nthreads < 0) return fail(HRRT_ERR_ARG, "nthreads must be >= 0");
+    if (nthreads > 0 && !ctx->dg.inc_ok)
         return fail(HRRT_ERR_UNSUPPORTED, "incremental back projection is not available for this geometry");
-    ctx->bp_incremental = incremental ? 1 : 0;
+    ctx->bp_threads = nthreads;
     ctx->drop_graphs();
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_get_backproject_schedule(const hrrt_ctx *ctx, int *nthreads) {
+    if (!ctx || !nthreads) return fail(HRRT_ERR_ARG, "null argument");
+    *nthreads = ctx->bp_threads;
     return HRRT_OK;
 }
 

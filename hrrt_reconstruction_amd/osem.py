@@ -249,8 +249,17 @@ class Osem3D(Layout):
         _lib.call("hrrt_back_proj3d_incremental", self.ctx, _p(prj), _p(ima), int(view), _stream())
         return ima
 
-    def set_backproject_algorithm(self, incremental: bool):
-        _lib.call("hrrt_set_backproject_algorithm", self.ctx, int(bool(incremental)))
+    def set_backproject_schedule(self, nthreads: int):
+        """The reference's -T nthreads: per subset, the first npairs - npairs % nthreads view
+        pairs are back-projected incrementally (back_proj3d_thread1), the rest directly
+        (back_proj3d2); 0 = direct for every pair."""
+        _lib.call("hrrt_set_backproject_schedule", self.ctx, int(nthreads))
+
+    @property
+    def backproject_schedule(self) -> int:
+        v = C.c_int(0)
+        _lib.call("hrrt_get_backproject_schedule", self.ctx, C.byref(v))
+        return v.value
 
     def back_proj3d(self, prj, ima, view: int):
         _lib.call("hrrt_back_proj3d", self.ctx, _p(prj), _p(ima), int(view), _stream())

@@ -233,10 +233,14 @@ int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *
  * HRRT_ERR_UNSUPPORTED when a breakpoint of the geometry moves the plane range
  * by other than one plane (none does for the HRRT models). */
 int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream);
-/* Back projector used by the FP32 subset updates (hrrt_osem_subset, *_frames):
- * 0 = direct (back_proj3d2), 1 = incremental (back_proj3d_thread1).  The
- * precision form always uses the direct variant with compensated sums. */
-int hrrt_set_backproject_algorithm(hrrt_ctx *ctx, int incremental);
+/* Back-projection schedule of the FP32 subset updates (hrrt_osem_subset, *_frames) and of
+ * hrrt_calculate_normfac, the reference's -T nthreads (S3:3796-3825, 4174-4218): the first
+ * npairs - npairs % nthreads view pairs of each subset take the incremental projector
+ * (back_proj3d_thread1), the rest the direct one (back_proj3d2); 0 = direct for every pair.
+ * Default 1 (every pair incremental) when the geometry allows it, else 0.  The precision
+ * form always uses the direct projector with compensated sums. */
+int hrrt_set_backproject_schedule(hrrt_ctx *ctx, int nthreads);
+int hrrt_get_backproject_schedule(const hrrt_ctx *ctx, int *nthreads);
 /* update_estimate_w3 (S3:2474): est = prompt==0 ? 0 : (est+add > eps3 ? prompt/(est+add) : 1),
  * over the [2S][N] rows of one view pair. */
 int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float *d_add, const int16_t *d_prompt, void *stream);

@@ -53,7 +53,8 @@ def test_frames_subset_matches_single_frames(torch_dev, oracle_lib, geom, nframe
         g.osem_subset_frames(d_imgs, d_nf, d_pr, d_add, s)
         for f in range(nframes):
             g.osem_subset(singles[f], d_nf, d_pr[f], d_add[f], s)
-        o.osem_subset(ref0, corr, np.ascontiguousarray(nf[s]), adds[0], prompts[0], s, nthreads=0, workers=8)
+        o.osem_subset(ref0, corr, np.ascontiguousarray(nf[s]), adds[0], prompts[0], s,
+                      nthreads=g.backproject_schedule, workers=8)
         got = from_dev(d_imgs)
         for f in range(nframes):
             assert np.array_equal(got[f], from_dev(singles[f])), f"subset {s} frame {f}"

@@ -71,7 +71,7 @@ def test_back_proj3d_incremental_masked_columns(torch_dev, oracle_lib):
 def test_osem_subsets_incremental_bitexact(torch_dev, oracle_lib, geom, subsets):
     print(f"osem incremental geom={geom} subsets={subsets}", flush=True)
     o, g = _pair(geom, oracle_lib)
-    g.set_backproject_algorithm(True)
+    g.set_backproject_schedule(1)
     rng = np.random.default_rng(37)
     shape = (o.S, o.views, o.N)
     prompt = rng.integers(0, 4, shape).astype(np.int16)
@@ -95,7 +95,7 @@ def test_frames_incremental_match_single(torch_dev, oracle_lib):
     print("frames incremental geom=tiny nframes=2", flush=True)
     o = oracle_lib.Oracle(**GEOMS["tiny"])
     g = Osem3D(**GEOMS["tiny"])
-    g.set_backproject_algorithm(True)
+    g.set_backproject_schedule(1)
     rng = np.random.default_rng(47)
     shape = (o.S, o.views, o.N)
     nframes = 2

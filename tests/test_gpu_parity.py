@@ -154,22 +154,29 @@ def test_stage_sinograms_bitexact(torch_dev, oracle_lib, geom):
         assert (g.norm_zero_mask() == 0).sum() >= 1
 
 
-@pytest.mark.parametrize("geom", ["tiny", "tinyrb"])
-def test_calculate_normfac_bitexact(torch_dev, oracle_lib, geom):
+@pytest.mark.parametrize("geom,nthreads", [("tiny", 0), ("tiny", 2), ("tinyrb", 0), ("tinyrb", 1)])
+def test_calculate_normfac_bitexact(torch_dev, oracle_lib, geom, nthreads):
+    """Sensitivity images under the reference's thread schedule -T nthreads (0: direct only)."""
+    print(f"normfac geom={geom} nthreads={nthreads}", flush=True)
     o, g = _pair(geom, oracle_lib)
+    g.set_backproject_schedule(nthreads)
     rng = np.random.default_rng(29)
     _, _, norm_f, _, atten_f = _study_files(o, rng, scatter=False)
     inv = o.prepare_norm(norm_f, atten_f)
     g.set_norm_zero_mask(o.norm_zero_mask[: o.views // 2])
-    ref = o.calculate_normfac(inv.copy(), nthreads=0)
+    ref = o.calculate_normfac(inv.copy(), nthreads=nthreads)
     nf = g.calculate_normfac(to_dev(g.sino_to_paircol(inv)))
     assert np.array_equal(from_dev(nf), ref)
 
 
-@pytest.mark.parametrize("geom,subsets", [("tiny", (0, 8, 1)), ("tinyrb", (0, 8)), ("c1", (0, 8))])
-def test_osem_subsets_bitexact(torch_dev, oracle_lib, geom, subsets):
-    print(f"osem geom={geom} subsets={subsets}", flush=True)
+@pytest.mark.parametrize("geom,subsets,nthreads", [("tiny", (0, 8, 1), 0), ("tiny", (0, 8), 2), ("tinyrb", (0, 8), 0),
+                                                   ("c1", (0, 8), 0), ("c1", (0, 5), 3)])
+def test_osem_subsets_bitexact(torch_dev, oracle_lib, geom, subsets, nthreads):
+    """Subset updates under the reference's thread schedule -T nthreads: per subset the first
+    npairs - npairs % nthreads pairs incremental, the rest direct (0: direct only)."""
+    print(f"osem geom={geom} subsets={subsets} nthreads={nthreads}", flush=True)
     o, g = _pair(geom, oracle_lib)
+    g.set_backproject_schedule(nthreads)
     rng = np.random.default_rng(31)
     shape = (o.S, o.views, o.N)
     prompt = rng.integers(0, 4, shape).astype(np.int16)
@@ -181,7 +188,7 @@ def test_osem_subsets_bitexact(torch_dev, oracle_lib, geom, subsets):
     d_pr, d_add = to_dev(g.sino_to_paircol(prompt)), to_dev(g.sino_to_paircol(add))
     corr = o.image_zeros()
     for s in subsets:
-        o.osem_subset(img, corr, np.ascontiguousarray(nf_all[s]), add, prompt, s, nthreads=0, workers=8)
+        o.osem_subset(img, corr, np.ascontiguousarray(nf_all[s]), add, prompt, s, nthreads=nthreads, workers=8)
         g.osem_subset(d_img, d_nf, d_pr, d_add, s)
         out = from_dev(d_img)
         assert np.array_equal(out, img), f"subset {s}: max |diff| {np.abs(out - img).max()} rel {rel_l2(out, img)}"
