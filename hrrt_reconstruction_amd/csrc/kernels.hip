@@ -679,33 +679,67 @@ __global__ void __launch_bounds__(256) bp_inc_kernel(const DevGeom g, const Pair
 // batch, accumulated in pair order, fused with the image update
 // (pt_cal_updateimage S3:2033-2056).  Block (64, 4): x lanes over z chunks of 4.
 // ---------------------------------------------------------------------------
+// One pair's bilinear neighbourhood of R = A + B at the rotated position of (x, y):
+// the eight loads are issued here and consumed one pair later.
+struct RotTap {
+    float4 a1, a2, a3, a4, b1, b2, b3, b4;
+    float x1, y1;
+    bool ok;
+};
+
+__device__ __forceinline__ void rot_issue(const DevGeom &g, const float4 *__restrict__ A,
+                                          const float4 *__restrict__ B, int lp, float2 pnt, int q, RotTap &t) {
+    const int N = g.N, NQ = g.NQ;
+    const int xi = (int)pnt.x, yi = (int)pnt.y;
+    t.ok = !(xi < 0 || xi > N - 2 || yi < 0 || yi > N - 2);
+    t.x1 = pnt.x - xi;
+    t.y1 = pnt.y - yi;
+    const int xc = t.ok ? xi : 0, yc = t.ok ? yi : 0;      // in-bounds reads for a skipped pair
+    const size_t base = (size_t)lp * N * N * NQ + q;
+    const size_t i1 = base + ((size_t)xc * N + yc) * NQ, i2 = base + ((size_t)(xc + 1) * N + yc) * NQ;
+    t.a1 = A[i1]; t.b1 = B[i1];
+    t.a2 = A[i2]; t.b2 = B[i2];
+    t.a3 = A[i1 + NQ]; t.b3 = B[i1 + NQ];
+    t.a4 = A[i2 + NQ]; t.b4 = B[i2 + NQ];
+}
+
+// Sum over the batch's pairs, in pair order, of the bilinear interpolation of R = A + B
+// at the position the reference's rotation table gives (x, y) for the pair's view.
+// Software-pipelined: the table entry two pairs ahead and the neighbourhood one pair
+// ahead are in flight while a pair is accumulated.
 template <bool KAHAN>
 __device__ __forceinline__ float4 rotate_accumulate(const DevGeom &g, const PairBatch &b,
                                                     const float4 *__restrict__ A, const float4 *__restrict__ B,
                                                     int x, int y, int q) {
-    const int N = g.N, NQ = g.NQ;
+    const int N = g.N;
+    const size_t vstride = (size_t)N * N * b.view_stride;
+    const float2 *rt = g.rot + ((size_t)b.view0 * N + x) * N + y;
     float4 corr = f4(0.f), comp = f4(0.f);
+    RotTap cur, nxt;
+    float2 pn = b.npairs > 1 ? rt[vstride] : make_float2(0.f, 0.f);
+    rot_issue(g, A, B, 0, rt[0], q, cur);
     for (int lp = 0; lp < b.npairs; lp++) {
-        const int view = b.view0 + b.view_stride * lp;
-        const float2 pnt = g.rot[((size_t)view * N + x) * N + y];
-        const int xi = (int)pnt.x, yi = (int)pnt.y;
-        if (xi < 0 || xi > N - 2 || yi < 0 || yi > N - 2) continue;
-        const float x1 = pnt.x - xi, y1 = pnt.y - yi;
-        const float w1 = (1 - x1) * (1 - y1), w2 = x1 * (1 - y1), w3 = y1 * (1 - x1), w4 = y1 * x1;
-        const size_t base = (size_t)lp * N * N * NQ + q;
-        const size_t i1 = base + ((size_t)xi * N + yi) * NQ, i2 = base + ((size_t)(xi + 1) * N + yi) * NQ;
-        const size_t i3 = base + ((size_t)xi * N + yi + 1) * NQ, i4 = base + ((size_t)(xi + 1) * N + yi + 1) * NQ;
-        const float4 m1 = add4(A[i1], B[i1]), m2 = add4(A[i2], B[i2]);
-        const float4 m3 = add4(A[i3], B[i3]), m4 = add4(A[i4], B[i4]);
-        const float4 v = add4(add4(mul4(m1, w1), mul4(m2, w2)), add4(mul4(m3, w3), mul4(m4, w4)));
-        if (KAHAN) {
-            kahan_add(corr.x, comp.x, v.x);
-            kahan_add(corr.y, comp.y, v.y);
-            kahan_add(corr.z, comp.z, v.z);
-            kahan_add(corr.w, comp.w, v.w);
-        } else {
-            corr = add4(corr, v);
+        if (lp + 1 < b.npairs) {
+            const float2 p = pn;
+            if (lp + 2 < b.npairs) pn = rt[vstride * (lp + 2)];
+            rot_issue(g, A, B, lp + 1, p, q, nxt);
         }
+        if (cur.ok) {
+            const float x1 = cur.x1, y1 = cur.y1;
+            const float w1 = (1 - x1) * (1 - y1), w2 = x1 * (1 - y1), w3 = y1 * (1 - x1), w4 = y1 * x1;
+            const float4 m1 = add4(cur.a1, cur.b1), m2 = add4(cur.a2, cur.b2);
+            const float4 m3 = add4(cur.a3, cur.b3), m4 = add4(cur.a4, cur.b4);
+            const float4 v = add4(add4(mul4(m1, w1), mul4(m2, w2)), add4(mul4(m3, w3), mul4(m4, w4)));
+            if (KAHAN) {
+                kahan_add(corr.x, comp.x, v.x);
+                kahan_add(corr.y, comp.y, v.y);
+                kahan_add(corr.z, comp.z, v.z);
+                kahan_add(corr.w, comp.w, v.w);
+            } else {
+                corr = add4(corr, v);
+            }
+        }
+        cur = nxt;
     }
     return corr;
 }
