@@ -1124,8 +1124,8 @@ static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float 
 }
 
 // Launch shape: MAXT planes per thread; the planes of one ray column are split over
-// nsplit workgroups; each workgroup has just enough 64-lane waves (at most 16) for its
-// share of the planes and for the sampling of a row.
+// nsplit workgroups; each workgroup has enough 64-lane waves (a multiple of 4, at most
+// 16) for its share of the planes and for the sampling of a row.
 void forward_shape(const DevGeom &g, int &maxt, int &nsplit, int &threads, int &items) {
     maxt = g.fp_maxt;
     nsplit = g.fp_nsplit;
@@ -1133,8 +1133,12 @@ void forward_shape(const DevGeom &g, int &maxt, int &nsplit, int &threads, int &
     const int need = (g.fp_np + 1024 * maxt - 1) / (1024 * maxt);
     if (nsplit < need) nsplit = need;
     const int per = (g.fp_np + nsplit - 1) / nsplit;
-    threads = (((per + maxt - 1) / maxt) + 63) / 64 * 64;
-    if (threads < 64) threads = 64;
+    // waves in multiples of 4, one per SIMD each: with one workgroup per CU and a barrier per
+    // row, a SIMD holding one wave more than the others would set the pace of every row
+    int waves = ((per + maxt - 1) / maxt + 63) / 64;
+    waves = (waves + 3) / 4 * 4;
+    if (waves > 16) waves = 16;
+    threads = 64 * waves;
     if ((8 * g.NQ + threads - 1) / threads > 2) threads = ((8 * g.NQ + 1) / 2 + 63) / 64 * 64;
     items = (8 * g.NQ + threads - 1) / threads;
 }
