@@ -76,6 +76,9 @@ SIGNATURES = {
     "hrrt_back_proj3d_incremental": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_set_backproject_schedule": (_I, [_VP, _I]),
     "hrrt_get_backproject_schedule": (_I, [_VP, _VP]),
+    "hrrt_set_psf": (_I, [_VP, _I, C.c_float, C.c_float, C.c_float]),
+    "hrrt_get_psf_kernel": (_I, [_VP, _I, _VP, _VP]),
+    "hrrt_convolve3d": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_update_estimate_w3": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "hrrt_cal_updateimage": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "hrrt_normfac_calculate": (_I, [_VP, _VP, _VP, _VP]),

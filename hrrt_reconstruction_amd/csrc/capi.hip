@@ -3,6 +3,7 @@
 // that chain the kernels of kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
@@ -49,6 +50,25 @@ struct hrrt_ctx {
     float *h_img = nullptr, *h_prj = nullptr;  // device staging for the host shims
     int device = 0;
     int bp_threads = 0;              // back-projection schedule (the reference's -T): 0 = direct for every pair
+    // PSF resolution model (-B): host kernels (x, y, z), normz, their device copies, and three
+    // scratch volumes (the blurred image, the correction, the x-pass buffer), zeroed once so
+    // that voxels outside the FOV, which the blur never writes, read as zero
+    bool psf_on = false;
+    std::vector<float> psf_k[3], psf_normz;
+    float *d_psf = nullptr;          // [taps x][taps y][normz]
+    float *psf_img = nullptr, *psf_corr = nullptr, *psf_buf = nullptr;
+    int ensure_psf_scratch() {
+        int r;
+        if ((r = zalloc((void **)&psf_img, sizeof(float) * vol_floats()))) return r;
+        if ((r = zalloc((void **)&psf_corr, sizeof(float) * vol_floats()))) return r;
+        if ((r = zalloc((void **)&psf_buf, sizeof(float) * vol_floats()))) return r;
+        return HRRT_OK;
+    }
+    hipError_t convolve3d(const float *in, float *out, hipStream_t s) const {
+        const int hx = ((int)psf_k[0].size() - 1) / 2, hy = ((int)psf_k[1].size() - 1) / 2;
+        return hrrt::launch_convolve3d(dg, in, out, psf_buf, d_psf, hx, d_psf + psf_k[0].size(), hy,
+                                       d_psf + psf_k[0].size() + psf_k[1].size(), geo.z_min, geo.z_max, s);
+    }
     // optional per-kernel timing of hrrt_osem_subset (HIP events on the launch stream)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -164,7 +184,7 @@ struct hrrt_ctx {
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
         void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, (void *)dg.nzmask, f_ratio, fA, fB,
-                     ll_part};
+                     ll_part, d_psf, psf_img, psf_corr, psf_buf};
         for (void *p : s)
             if (p) (void)hipFree(p);
     }
@@ -637,11 +657,18 @@ extern "C" int hrrt_calculate_normfac(hrrt_ctx *ctx, const float *d_invnorm, flo
     int r;
     if ((r = ctx->ensure_subset_scratch())) return r;
     const hrrt::Geometry &G = ctx->geo;
+    if (ctx->psf_on && (r = ctx->ensure_psf_scratch())) return r;
     for (int s = 0; s < G.p.subsets; s++) {
         hrrt::PairBatch b{s, G.p.subsets, ctx->npairs()};
+        float *nf = d_normfac + (size_t)s * ctx->vol_floats();
         HIP_TRY(backproject_scheduled(ctx, b, d_invnorm, 1, 1, ctx->bufA, ctx->bufB, S(stream)));
-        HIP_TRY(hrrt::launch_rotate_sens(ctx->dg, b, ctx->bufA, ctx->bufB, d_normfac + (size_t)s * ctx->vol_floats(),
-                                         0, S(stream)));
+        if (ctx->psf_on) {   // the sensitivity is blurred before inversion (S3:3835-3840)
+            HIP_TRY(hrrt::launch_rotate_sum(ctx->dg, b, ctx->bufA, ctx->bufB, nf, S(stream)));
+            HIP_TRY(ctx->convolve3d(nf, nf, S(stream)));
+            HIP_TRY(hrrt::launch_normfac_invert(ctx->dg, nf, nf, S(stream)));
+        } else {
+            HIP_TRY(hrrt::launch_rotate_sens(ctx->dg, b, ctx->bufA, ctx->bufB, nf, 0, S(stream)));
+        }
     }
     return HRRT_OK;
 }
@@ -665,16 +692,29 @@ extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_no
     if (isubset < 0 || isubset >= G.p.subsets) return fail(HRRT_ERR_ARG, "subset %d out of range", isubset);
     int r;
     if ((r = ctx->ensure_subset_scratch())) return r;
+    if (ctx->psf_on && (r = ctx->ensure_psf_scratch())) return r;
     hrrt::PairBatch b{isubset, G.p.subsets, ctx->npairs()};
     ctx->mark(0, S(stream));
     hrrt::FpIO io;
     io.prompt = d_prompt;
     io.add = d_add;
-    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->sub_ratio, io, hrrt::FP_RATIO, S(stream)));
+    const float *fp_image = d_image;
+    if (ctx->psf_on) {   // the forward projector reads the blurred image (S3:4172, 4202)
+        HIP_TRY(ctx->convolve3d(d_image, ctx->psf_img, S(stream)));
+        fp_image = ctx->psf_img;
+    }
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, fp_image, ctx->sub_ratio, io, hrrt::FP_RATIO, S(stream)));
     ctx->mark(1, S(stream));
     HIP_TRY(backproject_scheduled(ctx, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
     ctx->mark(2, S(stream));
-    HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, 0, S(stream)));
+    if (ctx->psf_on) {   // the correction is blurred before the update (S3:4229)
+        HIP_TRY(hrrt::launch_rotate_sum(ctx->dg, b, ctx->bufA, ctx->bufB, ctx->psf_corr, S(stream)));
+        HIP_TRY(ctx->convolve3d(ctx->psf_corr, ctx->psf_corr, S(stream)));
+        HIP_TRY(hrrt::launch_update_image(ctx->dg, d_image, ctx->psf_corr, d_normfac_subset, S(stream)));
+    } else {
+        HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, 0,
+                                           S(stream)));
+    }
     ctx->mark(-1, S(stream));
     return HRRT_OK;
 }
@@ -729,6 +769,8 @@ static int frames_subset_launch(hrrt_ctx *ctx, int nframes, float *d_images, con
 static int frames_check(hrrt_ctx *ctx, int nframes, const void *a, const void *b, const void *c, const void *d) {
     if (!ctx || !a || !b || !c || !d) return fail(HRRT_ERR_ARG, "null argument");
     if (nframes < 1) return fail(HRRT_ERR_ARG, "nframes must be >= 1");
+    if (ctx->psf_on)
+        return fail(HRRT_ERR_UNSUPPORTED, "the PSF model is available for the single-frame FP32 path only");
     const long long blocks = (long long)ctx->geo.N / 2 * 64 * ctx->npairs() * nframes;   // forward grid bound
     if (blocks > 0x7fffffffLL || (long long)ctx->npairs() * nframes > 65535)
         return fail(HRRT_ERR_ARG, "too many frames for one launch (%d)", nframes);
@@ -816,6 +858,8 @@ extern "C" int hrrt_to_half(const float *d_in, uint16_t *d_out, long long n, voi
 extern "C" int hrrt_calculate_normfac_mixed(hrrt_ctx *ctx, const uint16_t *d_invnorm16, float *d_normfac,
                                             void *stream) {
     if (!ctx || !d_invnorm16 || !d_normfac) return fail(HRRT_ERR_ARG, "null argument");
+    if (ctx->psf_on)
+        return fail(HRRT_ERR_UNSUPPORTED, "the PSF model is available for the single-frame FP32 path only");
     int r;
     if ((r = ctx->ensure_subset_scratch())) return r;
     const hrrt::Geometry &G = ctx->geo;
@@ -959,6 +1003,92 @@ extern "C" int hrrt_set_backproject_schedule(hrrt_ctx *ctx, int nthreads) {
         return fail(HRRT_ERR_UNSUPPORTED, "incremental back projection is not available for this geometry");
     ctx->bp_threads = nthreads;
     ctx->drop_graphs();
+    return HRRT_OK;
+}
+
+// make1d_psf_filter psf.cpp:20-70: the sum of two Gaussians evaluated in double, stored in float
+// and normalised in float; fwhm1 == 0 selects the HRRT (model 999) defaults
+static int psf_kernel(float fwhm1, float fwhm2, float ratio, float pixel_size, std::vector<float> &k) {
+    if (fwhm1 == 0.0f) {
+        fwhm1 = (float)(0.9 * sqrt(8.0 * log(2.0)));
+        fwhm2 = (float)(2.5 * sqrt(8.0 * log(2.0)));
+        ratio = (float)0.05;
+    }
+    if (!(fwhm1 > 0.0f)) return -1;
+    const bool two = fwhm2 > 0.0f && ratio > 0.0f;
+    const int size = 2 * ((int)(1.5 * (two ? fwhm2 : fwhm1) * sqrt(log(10.0) / log(2.)) / pixel_size / 2.0)) + 1;
+    if (size < 1 || size > 1023) return -1;
+    k.assign(size, 0.0f);
+    const float sigma1 = (float)(fwhm1 / (sqrt(8.0 * log(2.0)) * pixel_size));
+    const float sigma2 = (float)(fwhm2 / (sqrt(8.0 * log(2.0)) * pixel_size));
+    const int bin0 = (size - 1) / 2;
+    float norm = 0.0f;
+    for (int bin = 0; bin < size; bin++) {
+        k[bin] = (float)exp(-0.5 * (pow((double)(bin - bin0) / sigma1, 2.0)));
+        if (two) k[bin] = (float)(k[bin] + ratio * exp(-0.5 * (pow((double)(bin - bin0) / sigma2, (double)2.0))));
+        norm += k[bin];
+    }
+    for (int bin = 0; bin < size; bin++) k[bin] /= norm;
+    return size;
+}
+
+extern "C" int hrrt_set_psf(hrrt_ctx *ctx, int enable, float fwhm1, float fwhm2, float ratio) {
+    if (!ctx) return fail(HRRT_ERR_ARG, "null context");
+    ctx->drop_graphs();
+    if (!enable) {
+        ctx->psf_on = false;
+        return HRRT_OK;
+    }
+    const hrrt::Geometry &G = ctx->geo;
+    std::vector<float> k[3];
+    const float px[3] = {G.x_size, G.y_size, G.z_size};
+    for (int a = 0; a < 3; a++)
+        if (psf_kernel(fwhm1, fwhm2, ratio, px[a], k[a]) < 1)
+            return fail(HRRT_ERR_ARG, "invalid PSF parameters (fwhm1 %g, fwhm2 %g, ratio %g)", fwhm1, fwhm2, ratio);
+    // the z pass reads the z kernel at the y kernel's indices (psf.cpp:356-374)
+    if (k[2].size() < k[1].size())
+        return fail(HRRT_ERR_UNSUPPORTED, "z kernel (%zu taps) shorter than the y kernel (%zu taps)", k[2].size(),
+                    k[1].size());
+    const int hy = ((int)k[1].size() - 1) / 2;
+    std::vector<float> normz(G.z4, 0.0f);
+    for (int mask = -hy; mask < hy; mask++)
+        for (int z = 0; z < G.z4; z++) {
+            const int m2 = z + mask;
+            if (m2 < G.z_min || m2 >= G.z_max) continue;
+            normz[z] += k[2][mask + hy];
+        }
+    for (int z = 0; z < G.z4; z++)
+        if (normz[z] > 0) normz[z] = 1 / normz[z];
+    std::vector<float> packed(k[0]);
+    packed.insert(packed.end(), k[1].begin(), k[1].end());
+    packed.insert(packed.end(), normz.begin(), normz.end());
+    if (ctx->d_psf) (void)hipFree(ctx->d_psf);
+    ctx->d_psf = nullptr;
+    if (hipMalloc((void **)&ctx->d_psf, sizeof(float) * packed.size()) != hipSuccess)
+        return fail(HRRT_ERR_NOMEM, "hipMalloc of the PSF kernels failed");
+    HIP_TRY(hipMemcpy(ctx->d_psf, packed.data(), sizeof(float) * packed.size(), hipMemcpyHostToDevice));
+    for (int a = 0; a < 3; a++) ctx->psf_k[a] = k[a];
+    ctx->psf_normz = normz;
+    ctx->psf_on = true;
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_get_psf_kernel(const hrrt_ctx *ctx, int axis, float *taps, int *size) {
+    if (!ctx || !size) return fail(HRRT_ERR_ARG, "null argument");
+    if (axis < 0 || axis > 2) return fail(HRRT_ERR_ARG, "axis %d out of range", axis);
+    if (!ctx->psf_on) return fail(HRRT_ERR_ARG, "the PSF model is not enabled");
+    const std::vector<float> &k = ctx->psf_k[axis];
+    if (taps) std::memcpy(taps, k.data(), sizeof(float) * k.size());
+    *size = (int)k.size();
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_convolve3d(hrrt_ctx *ctx, const float *d_in, float *d_out, void *stream) {
+    if (!ctx || !d_in || !d_out) return fail(HRRT_ERR_ARG, "null argument");
+    if (!ctx->psf_on) return fail(HRRT_ERR_ARG, "the PSF model is not enabled (hrrt_set_psf)");
+    int r;
+    if ((r = ctx->ensure_psf_scratch())) return r;
+    HIP_TRY(ctx->convolve3d(d_in, d_out, S(stream)));
     return HRRT_OK;
 }
 

@@ -98,6 +98,13 @@ hipError_t launch_estimate_to_paircol(const DevGeom &g, float *prj, float *pairc
 hipError_t launch_ratio_w3(const DevGeom &g, float *est, const float *add, const int16_t *prompt, hipStream_t s);
 hipError_t launch_update_image(const DevGeom &g, float *image, float *corr, const float *nf, hipStream_t s);
 hipError_t launch_normfac_invert(const DevGeom &g, const float *sens, float *out, hipStream_t s);
+// rotation of the batch's pairs summed into corr (FOV voxels), no update
+hipError_t launch_rotate_sum(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB, float *corr,
+                             hipStream_t s);
+// PSF blur convolve3d: x pass in -> buf, y/z pass buf -> out (out may be in); taps of the x and
+// y kernels with half-widths hx, hy; normz [Z4]; the z pass keeps planes inside [z_min, z_max)
+hipError_t launch_convolve3d(const DevGeom &g, const float *in, float *out, float *buf, const float *taps_x, int hx,
+                             const float *taps_y, int hy, const float *normz, int z_min, int z_max, hipStream_t s);
 hipError_t launch_init_image(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s);
 hipError_t launch_end_iteration(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s);
 hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *delayed_f, const float *norm_f,

@@ -771,6 +771,89 @@ __global__ void __launch_bounds__(256) rot_update_kernel(const DevGeom g, const 
                            update1(im.z, corr.z, n.z, g.image_max), update1(im.w, corr.w, n.w, g.image_max));
 }
 
+// Rotation sum only (rotate_image_buf into a zeroed correction, PJ:1701-1725), for the
+// paths that filter the correction or the sensitivity before using it.
+__global__ void __launch_bounds__(256) rot_sum_kernel(const DevGeom g, const PairBatch b,
+                                                      const float4 *__restrict__ A, const float4 *__restrict__ B,
+                                                      float4 *__restrict__ corr) {
+    int x, yg;
+    rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
+    const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
+    if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
+    corr[((size_t)x * g.N + y) * g.NQ + q] = rotate_accumulate<false>(g, b, A, B, x, y, q);
+}
+
+// ---------------------------------------------------------------------------
+// PSF resolution model (convolve3d psf.cpp:441-476).  The x pass (convolvex
+// psf.cpp:168-219) sums, for a FOV voxel, the taps mask = -h .. h-1 whose column
+// lies inside the row's FOV and divides by their sum unless it is 0 or 1.  The
+// y/z pass (convolveyz psf.cpp:324-439) sums the y taps the same way into a
+// z column kept in LDS, then every plane z < z_pixels takes, with the y kernel,
+// the planes z + mask inside [z_min, z_max), and is scaled by normz(z) * (1/sum
+// of the y taps).  Only FOV voxels are written, as in the reference.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) psf_x_kernel(const DevGeom g, const float4 *__restrict__ in,
+                                                    float4 *__restrict__ out, const float *__restrict__ taps, int h) {
+    int x, yg;
+    rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
+    const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
+    if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
+    const int lo = g.cyl_lo[y], hi = g.cyl_hi[y];
+    float4 acc = f4(0.f);
+    float norm = 0.0f;
+    for (int mask = -h; mask < h; mask++) {
+        const int m2 = x + mask;
+        if (m2 < lo || m2 >= hi) continue;
+        const float f = taps[mask + h];
+        norm = norm + f;
+        acc = add4(acc, mul4(in[((size_t)m2 * g.N + y) * g.NQ + q], f));
+    }
+    if (norm != 1.0f && norm != 0.0f) acc = mul4(acc, 1.0f / norm);
+    out[((size_t)x * g.N + y) * g.NQ + q] = acc;
+}
+
+__global__ void __launch_bounds__(256) psf_yz_kernel(const DevGeom g, const float4 *__restrict__ in,
+                                                     float *__restrict__ out, const float *__restrict__ taps, int h,
+                                                     const float *__restrict__ normz, int z_min, int z_max) {
+    __shared__ float col[4][256];
+    int x, yg;
+    rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
+    const int q = threadIdx.x, ty = threadIdx.y, y = yg * blockDim.y + ty;
+    const int lo = g.cyl_lo[x], hi = g.cyl_hi[x];
+    const bool fov = y >= lo && y < hi;
+    float norm = 0.0f;
+    if (fov) {
+        float4 acc = f4(0.f);
+        for (int mask = -h; mask < h; mask++) {
+            const int m2 = y + mask;
+            if (m2 < lo || m2 >= hi) continue;
+            const float f = taps[mask + h];
+            norm = norm + f;
+            if (q < g.NQ) acc = add4(acc, mul4(in[((size_t)x * g.N + m2) * g.NQ + q], f));
+        }
+        if (q < g.NQ) {
+            col[ty][4 * q + 0] = acc.x;
+            col[ty][4 * q + 1] = acc.y;
+            col[ty][4 * q + 2] = acc.z;
+            col[ty][4 * q + 3] = acc.w;
+        }
+    }
+    __syncthreads();
+    if (!fov) return;
+    float *o = out + ((size_t)x * g.N + y) * g.Z4;
+    for (int z = q; z < g.Z4; z += 64) {
+        float v = 0.0f;
+        if (norm != 0.0f) {
+            for (int mask = -h; mask < h; mask++) {
+                const int zz = z + mask;
+                if (z < g.Z && zz >= z_min && zz < z_max) v = v + col[ty][zz] * taps[mask + h];
+            }
+            v = v * (normz[z] * (1.0f / norm));
+        }
+        o[z] = v;
+    }
+}
+
 __device__ __forceinline__ float invert1(float s, float eps1) { return fabsf(s) > eps1 ? (1.0f / s) : 0.0f; }
 
 template <bool KAHAN>
@@ -1255,6 +1338,25 @@ hipError_t launch_ratio_w3(const DevGeom &g, float *est, const float *add, const
 
 hipError_t launch_update_image(const DevGeom &g, float *image, float *corr, const float *nf, hipStream_t s) {
     hipLaunchKernelGGL(update_image_kernel, dim3(g.N), dim3(64, 4), 0, s, g, image, corr, nf);
+    return hipGetLastError();
+}
+
+hipError_t launch_rotate_sum(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB, float *corr,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(rot_sum_kernel, rot_grid(g), dim3(64, 4), 0, s, g, b, reinterpret_cast<const float4 *>(bufA),
+                       reinterpret_cast<const float4 *>(bufB), reinterpret_cast<float4 *>(corr));
+    return hipGetLastError();
+}
+
+hipError_t launch_convolve3d(const DevGeom &g, const float *in, float *out, float *buf, const float *taps_x, int hx,
+                             const float *taps_y, int hy, const float *normz, int z_min, int z_max, hipStream_t s) {
+    if (g.Z4 > 256 || g.NQ > 64 || z_min < 0 || z_max > g.Z) return hipErrorInvalidValue;   // one z column per wave
+    hipLaunchKernelGGL(psf_x_kernel, rot_grid(g), dim3(64, 4), 0, s, g, reinterpret_cast<const float4 *>(in),
+                       reinterpret_cast<float4 *>(buf), taps_x, hx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(psf_yz_kernel, rot_grid(g), dim3(64, 4), 0, s, g, reinterpret_cast<const float4 *>(buf), out,
+                       taps_y, hy, normz, z_min, z_max);
     return hipGetLastError();
 }
 

@@ -255,6 +255,24 @@ class Osem3D(Layout):
         (back_proj3d2); 0 = direct for every pair."""
         _lib.call("hrrt_set_backproject_schedule", self.ctx, int(nthreads))
 
+    # ---- PSF resolution model (-B, psf.cpp) ----
+    def set_psf(self, fwhm1: float = 0.0, fwhm2: float = 0.0, ratio: float = 0.0, enable: bool = True):
+        """The reference's -B fwhm1,fwhm2,ratio (mm); fwhm1 = 0 selects the HRRT defaults."""
+        _lib.call("hrrt_set_psf", self.ctx, int(bool(enable)), float(fwhm1), float(fwhm2), float(ratio))
+
+    def psf_kernel(self, axis: int) -> np.ndarray:
+        n = C.c_int(0)
+        _lib.call("hrrt_get_psf_kernel", self.ctx, int(axis), None, C.byref(n))
+        out = np.zeros(n.value, np.float32)
+        _lib.call("hrrt_get_psf_kernel", self.ctx, int(axis), out.ctypes.data_as(C.c_void_p), C.byref(n))
+        return out
+
+    def convolve3d(self, image, out=None):
+        """convolve3d (psf.cpp:441): blur an image [X][Y][Z4]; out may be image."""
+        out = torch.zeros_like(image) if out is None else out
+        _lib.call("hrrt_convolve3d", self.ctx, _p(image), _p(out), _stream())
+        return out
+
     @property
     def backproject_schedule(self) -> int:
         v = C.c_int(0)

@@ -241,6 +241,19 @@ int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *d_ima, int 
  * form always uses the direct projector with compensated sums. */
 int hrrt_set_backproject_schedule(hrrt_ctx *ctx, int nthreads);
 int hrrt_get_backproject_schedule(const hrrt_ctx *ctx, int *nthreads);
+/* PSF resolution model, the reference's -B fwhm1,fwhm2,ratio (S3:3327-3330, 4521-4540;
+ * psf.cpp:20-70): a kernel per axis, the sum of two Gaussians of FWHM fwhm1 and fwhm2 (mm)
+ * weighted 1 : ratio, sampled at the x, y and z pixel sizes; fwhm1 == 0 selects the HRRT
+ * defaults.  When enabled, hrrt_osem_subset forward-projects the blurred image and blurs
+ * the correction before the update (S3:4172-4229), and hrrt_calculate_normfac blurs each
+ * sensitivity image before inverting it (S3:3835-3840).  enable = 0 turns it off.
+ * The batched-frame and precision forms return HRRT_ERR_UNSUPPORTED while it is on. */
+int hrrt_set_psf(hrrt_ctx *ctx, int enable, float fwhm1, float fwhm2, float ratio);
+/* Kernel taps of axis 0/1/2 (x/y/z) into taps (host, may be NULL); *size receives the count. */
+int hrrt_get_psf_kernel(const hrrt_ctx *ctx, int axis, float *taps, int *size);
+/* convolve3d (psf.cpp:441-476) of an image [X][Y][Z4]; d_out may be d_in.  Only voxels
+ * inside the FOV are written. */
+int hrrt_convolve3d(hrrt_ctx *ctx, const float *d_in, float *d_out, void *stream);
 /* update_estimate_w3 (S3:2474): est = prompt==0 ? 0 : (est+add > eps3 ? prompt/(est+add) : 1),
  * over the [2S][N] rows of one view pair. */
 int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float *d_add, const int16_t *d_prompt, void *stream);

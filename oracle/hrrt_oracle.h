@@ -140,6 +140,28 @@ void     og_prepare_w3(const og_geom *g, const int16_t *prompt_file, const float
 /* Gather one view pair's rows from a [S][views][N] sinogram into [2S][N] (largeprj[v] S3:2149). */
 void     og_gather_pair_f(const og_geom *g, const float *sino, int view, float *out);
 void     og_gather_pair_s(const og_geom *g, const int16_t *sino, int view, int16_t *out);
+/* ---- PSF resolution modelling (-B, psf.cpp) ---- */
+typedef struct {
+    int    size[3];          /* taps of the x, y, z kernels (odd)                 */
+    float *k[3];             /* kernels (make1d_psf_filter psf.cpp:20-70)         */
+    float *normz;            /* [z4] 1 / sum of z-kernel taps inside [z_min, z_max) */
+} og_psf;
+/* fwhm1 == 0 selects the HRRT defaults (psf.cpp:25-29); NULL if fwhm1 < 0 or if the
+ * z kernel is shorter than the y kernel (the reference would read past it) */
+og_psf  *og_psf_create(const og_geom *g, float fwhm1, float fwhm2, float ratio);
+void     og_psf_destroy(og_psf *p);
+int      og_psf_kernel(const og_psf *p, int axis, float *out);   /* returns the size */
+/* convolve3d psf.cpp:441-476: out may alias image; buf [X][Y][Z4] scratch */
+void     og_convolve3d(const og_geom *g, const og_psf *p, const float *image, float *out, float *buf);
+/* CalculateNormfac / CalculateOsem3d with -B (S3:3835-3840, 4172-4229): the sensitivity
+ * is blurred before inversion; the forward projector reads the blurred image and the
+ * correction is blurred before the update */
+void     og_calculate_normfac_psf(const og_geom *g, const og_psf *p, float *invnorm, float *normfac_all,
+                                  int nthreads);
+void     og_osem_subset_psf(const og_geom *g, const og_psf *p, float *image, float *correction,
+                            const float *normfac_subset, const float *add, const int16_t *prompt, int isubset,
+                            int nthreads, int workers);
+
 /* CalculateNormfac S3:3760: normfac_all [subsets][X][Y][Z4] receives 1/sens per subset.
  * invnorm: [S][views][N] (= 1/(N*A)); it is rescaled in place, as the reference does.
  * nthreads selects the reference's split between the *_thread1 and *2 variants

@@ -64,9 +64,8 @@ class Geom(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"], check=True)
+    """Compile liboracle.so with the committed Makefile (make rebuilds it when a source changed)."""
+    subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []) + ["liboracle.so"], check=True)
     return _LIB_PATH
 
 
@@ -105,6 +104,14 @@ def lib():
         L.og_gather_pair_s.argtypes = [G, vp, C.c_int, vp]
         L.og_calculate_normfac.argtypes = [G, vp, vp, C.c_int]
         L.og_osem_subset.argtypes = [G, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+        L.og_psf_create.restype = vp
+        L.og_psf_create.argtypes = [G, C.c_float, C.c_float, C.c_float]
+        L.og_psf_destroy.argtypes = [vp]
+        L.og_psf_kernel.restype = C.c_int
+        L.og_psf_kernel.argtypes = [vp, C.c_int, vp]
+        L.og_convolve3d.argtypes = [G, vp, vp, vp, vp]
+        L.og_calculate_normfac_psf.argtypes = [G, vp, vp, vp, C.c_int]
+        L.og_osem_subset_psf.argtypes = [G, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -147,6 +154,9 @@ class Oracle:
         self.subsets = self.params.subsets
 
     def close(self):
+        if getattr(self, "psf", None):
+            self.L.og_psf_destroy(self.psf)
+            self.psf = None
         if self.gp:
             self.L.og_destroy(self.gp)
             self.gp = None
@@ -349,3 +359,33 @@ class Oracle:
     def osem_subset(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
         self.L.og_osem_subset(self.gp, _ptr(image), _ptr(correction), _ptr(normfac_subset), _ptr(add),
                               _ptr(prompt), isubset, nthreads, workers)
+
+    # ---- PSF resolution modelling (-B) ----
+    def set_psf(self, fwhm1=0.0, fwhm2=0.0, ratio=0.0):
+        """Build the x, y, z kernels (make1d_psf_filter); fwhm1 == 0 selects the HRRT defaults."""
+        if getattr(self, "psf", None):
+            self.L.og_psf_destroy(self.psf)
+        self.psf = self.L.og_psf_create(self.gp, fwhm1, fwhm2, ratio)
+        if not self.psf:
+            raise ValueError("invalid PSF parameters")
+
+    def psf_kernel(self, axis):
+        n = self.L.og_psf_kernel(self.psf, axis, None)
+        out = np.zeros(n, np.float32)
+        self.L.og_psf_kernel(self.psf, axis, _ptr(out))
+        return out
+
+    def convolve3d(self, image, out=None):
+        out = self.image_zeros() if out is None else out
+        buf = self.image_zeros()
+        self.L.og_convolve3d(self.gp, self.psf, _ptr(image), _ptr(out), _ptr(buf))
+        return out
+
+    def calculate_normfac_psf(self, invnorm, nthreads=0):
+        nf = np.zeros((self.subsets, self.N, self.N, self.Z4), np.float32)
+        self.L.og_calculate_normfac_psf(self.gp, self.psf, _ptr(invnorm), _ptr(nf), nthreads)
+        return nf
+
+    def osem_subset_psf(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
+        self.L.og_osem_subset_psf(self.gp, self.psf, _ptr(image), _ptr(correction), _ptr(normfac_subset),
+                                  _ptr(add), _ptr(prompt), isubset, nthreads, workers)

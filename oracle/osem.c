@@ -747,3 +747,36 @@ void og_osem_subset(const og_geom *g, float *image, float *correction, const flo
     run_subset_pairs(g, isubset, nthreads, workers, 0, NULL, image, add, prompt, correction);
     og_update_image(g, image, correction, normfac_subset);
 }
+
+void og_calculate_normfac_psf(const og_geom *g, const og_psf *p, float *invnorm, float *normfac_all, int nthreads)
+{   /* CalculateNormfac with -B: convolve3d(normfac, normfac) before pt_normfac_calculate S3:3835-3840 */
+    const size_t V = (size_t)g->N * g->N * g->z4;
+    float *sens = (float *)malloc(sizeof(float) * V);
+    float *buf = (float *)calloc(V, sizeof(float));
+    int s;
+    for (s = 0; s < g->p.subsets; s++) {
+        memset(sens, 0, sizeof(float) * V);
+        run_subset_pairs(g, s, nthreads, nthreads > 0 ? nthreads : 1, 1, invnorm, NULL, NULL, NULL, sens);
+        og_convolve3d(g, p, sens, sens, buf);
+        memset(normfac_all + (size_t)s * V, 0, sizeof(float) * V);
+        og_normfac_invert(g, sens, normfac_all + (size_t)s * V);
+    }
+    free(buf);
+    free(sens);
+}
+
+void og_osem_subset_psf(const og_geom *g, const og_psf *p, float *image, float *correction,
+                        const float *normfac_subset, const float *add, const int16_t *prompt, int isubset,
+                        int nthreads, int workers)
+{   /* CalculateOsem3d with -B: image_psf for the forward projector S3:4172-4202, the
+     * correction blurred before the update S3:4229 */
+    const size_t V = (size_t)g->N * g->N * g->z4;
+    float *image_psf = (float *)calloc(V, sizeof(float));
+    float *buf = (float *)calloc(V, sizeof(float));
+    og_convolve3d(g, p, image, image_psf, buf);
+    run_subset_pairs(g, isubset, nthreads, workers, 0, NULL, image_psf, add, prompt, correction);
+    og_convolve3d(g, p, correction, correction, buf);
+    og_update_image(g, image, correction, normfac_subset);
+    free(buf);
+    free(image_psf);
+}

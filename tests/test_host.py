@@ -137,7 +137,7 @@ def test_null_arguments(lib):
     for name, (_, args) in lib.SIGNATURES.items():
         if name in ("hrrt_destroy", "hrrt_last_error", "hrrt_create", "hrrt_query_geometry"):
             continue
-        zero = [0 if a in (C.c_int, C.c_longlong) else None for a in args]
+        zero = [0 if a in (C.c_int, C.c_longlong) else (0.0 if a is C.c_float else None) for a in args]
         assert getattr(L, name)(*zero) == lib.HRRT_ERR_ARG, name
     L.hrrt_destroy(None)          # a no-op, like free(NULL)
 
