@@ -130,6 +130,34 @@ def cpu_baseline(g, torch, prompt, add, normfac, image, nsub, threads):
 DYN_GEOM = dict(x_pixels=128, span=9, maxdel=67)
 
 
+def psf_leg(torch, g, image0, normfac_psf, prompt, add, order, steps):
+    """The same subset updates with the PSF resolution model on (-B with the HRRT defaults):
+    blurred image into the forward projector, blurred correction before the update."""
+    g.set_psf()
+    image = image0.clone()
+    tmp = torch.zeros_like(image)
+    for i in range(2):
+        g.osem_subset(image, normfac_psf, prompt, add, order[i])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        g.osem_subset(image, normfac_psf, prompt, add, order[(2 + i) % g.subsets])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        g.convolve3d(image, out=tmp)
+    e1.record()
+    torch.cuda.synchronize()
+    taps = [len(g.psf_kernel(a)) for a in range(3)]
+    g.set_psf(enable=False)
+    log(f"PSF: {steps} subset updates in {dt:.3f} s, convolve3d {e0.elapsed_time(e1) / 10:.3f} ms")
+    return {"subset_iters_per_s": round(steps / dt, 3), "ms_per_subset": round(1e3 * dt / steps, 4),
+            "convolve3d_ms": round(e0.elapsed_time(e1) / 10, 4), "kernel_taps_xyz": taps,
+            "note": "two convolve3d per subset (image, correction), sensitivity blurred before inversion"}
+
+
 def batched_frames(torch, nframes, iterations, seed):
     """Dynamic study in batched mode: every frame's subsets run in one launch per kernel, each subset
     of an iteration replayed from a HIP graph.  FP32 form and FP16-input precision form."""
@@ -239,6 +267,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the forward-projector launch-shape autotuner")
     ap.add_argument("--frames", type=int, default=24, help="frames of the batched dynamic study (0 = skip)")
     ap.add_argument("--frame-iterations", type=int, default=2)
+    ap.add_argument("--psf-steps", type=int, default=16,
+                    help="timed subset updates with the PSF resolution model (-B, HRRT defaults; 0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -273,6 +303,11 @@ def main():
     torch.cuda.synchronize()
     normfac_s = time.perf_counter() - t0
     log(f"sensitivity images in {normfac_s:.2f} s")
+    normfac_psf = None
+    if args.psf_steps > 0:
+        g.set_psf()
+        normfac_psf = g.calculate_normfac(invnorm)
+        g.set_psf(enable=False)
     del study, invnorm
     torch.cuda.empty_cache()
 
@@ -317,6 +352,9 @@ def main():
                "sample": f"{args.cpu_subsets} OSEM subset updates of the same workload after one warm-up "
                          f"subset, reference thread schedule T={args.cpu_threads} ({dt:.1f} s)"}
 
+    psf = None
+    if args.psf_steps > 0:
+        psf = psf_leg(torch, g, image0, normfac_psf, prompt, add, order, args.psf_steps)
     batched = None
     if args.frames > 0:
         batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
@@ -346,6 +384,7 @@ def main():
         "kernels": {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 4), "launches": v[1],
                         "alg_bytes": alg[k]} for k, v in kt.items()},
         "cpu_baseline": cpu,
+        "psf": psf,
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

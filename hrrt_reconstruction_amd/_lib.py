@@ -44,6 +44,13 @@ class Dims(C.Structure):
 
 # every entry point of include/hrrt_osem.h with its ctypes signature
 _VP, _I = C.c_void_p, C.c_int
+
+
+class ImageInfo(C.Structure):
+    """hrrt_image_info: the contents of the image's Interfile header."""
+    _fields_ = [("recontype", C.c_int), ("iterations", C.c_int)] + [
+        (n, C.c_char_p) for n in ("true_file", "prompt_file", "delayed_file", "norm_file", "atten_file",
+                                  "scatter_file", "sw_version", "sw_build_id")]
 SIGNATURES = {
     "hrrt_create": (_I, [C.POINTER(Params), C.POINTER(_VP)]),
     "hrrt_destroy": (None, [_VP]),
@@ -77,6 +84,12 @@ SIGNATURES = {
     "hrrt_set_backproject_schedule": (_I, [_VP, _I]),
     "hrrt_get_backproject_schedule": (_I, [_VP, _VP]),
     "hrrt_set_psf": (_I, [_VP, _I, C.c_float, C.c_float, C.c_float]),
+    "hrrt_image_to_flat": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "hrrt_image_from_flat": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "hrrt_write_image": (_I, [_VP, _VP, C.c_char_p, _I, _VP, _VP]),
+    "hrrt_read_image": (_I, [_VP, C.c_char_p, _VP, _I, _VP]),
+    "hrrt_write_normfac": (_I, [_VP, _VP, C.c_char_p, _VP]),
+    "hrrt_read_normfac": (_I, [_VP, C.c_char_p, _VP, _VP]),
     "hrrt_get_psf_kernel": (_I, [_VP, _I, _VP, _VP]),
     "hrrt_convolve3d": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_update_estimate_w3": (_I, [_VP, _VP, _VP, _VP, _VP]),

@@ -57,6 +57,7 @@ struct hrrt_ctx {
     std::vector<float> psf_k[3], psf_normz;
     float *d_psf = nullptr;          // [taps x][taps y][normz]
     float *psf_img = nullptr, *psf_corr = nullptr, *psf_buf = nullptr;
+    float *flat = nullptr;           // [z_pixels][N][N] staging of the flat image files
     int ensure_psf_scratch() {
         int r;
         if ((r = zalloc((void **)&psf_img, sizeof(float) * vol_floats()))) return r;
@@ -184,7 +185,7 @@ struct hrrt_ctx {
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
         void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, (void *)dg.nzmask, f_ratio, fA, fB,
-                     ll_part, d_psf, psf_img, psf_corr, psf_buf};
+                     ll_part, d_psf, psf_img, psf_corr, psf_buf, flat};
         for (void *p : s)
             if (p) (void)hipFree(p);
     }
@@ -1089,6 +1090,201 @@ extern "C" int hrrt_convolve3d(hrrt_ctx *ctx, const float *d_in, float *d_out, v
     int r;
     if ((r = ctx->ensure_psf_scratch())) return r;
     HIP_TRY(ctx->convolve3d(d_in, d_out, S(stream)));
+    return HRRT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Image and sensitivity files (SaveImage S3:4016-4096, write_flat_image FIO:300-333,
+// Init_Image -I S3:3970-3993, write_norm / read_norm FIO:360-427, write_image_header
+// write_image_header.cpp:67-319)
+// ---------------------------------------------------------------------------
+extern "C" int hrrt_image_to_flat(hrrt_ctx *ctx, const float *d_image, float *d_flat, int flip, void *stream) {
+    if (!ctx || !d_image || !d_flat) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_flat_image(ctx->dg, d_image, d_flat, 1, flip ? 1 : 0, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_image_from_flat(hrrt_ctx *ctx, const float *d_flat, float *d_image, int flip, void *stream) {
+    if (!ctx || !d_image || !d_flat) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_flat_image(ctx->dg, d_flat, d_image, 0, flip ? 1 : 0, S(stream)));
+    return HRRT_OK;
+}
+
+static size_t flat_floats(const hrrt_ctx *ctx) { return (size_t)ctx->geo.z_pixels * ctx->geo.N * ctx->geo.N; }
+
+static const char *base_name(const char *path) {
+    const char *p = std::strrchr(path, '/');
+    return p ? p + 1 : path;
+}
+
+// scatter fraction from the scatter file's Interfile header (<name>.h33), 0 when absent
+static float scatter_fraction(const char *scatter_file) {
+    std::string hdr(scatter_file);
+    const size_t dot = hdr.rfind('.');
+    if (dot != std::string::npos && hdr.find('/', dot) == std::string::npos) hdr.resize(dot);
+    hdr += ".h33";
+    FILE *fp = std::fopen(hdr.c_str(), "rt");
+    if (!fp) return 0.0f;
+    char line[1024];
+    float scf = 0.0f;
+    while (std::fgets(line, sizeof line, fp)) {
+        if (!std::strstr(line, "scatter fraction")) continue;
+        const char *p = std::strstr(line, ":=");
+        if (p && std::sscanf(p + 2, "%g", &scf) == 1) break;
+    }
+    std::fclose(fp);
+    return scf;
+}
+
+static int write_header(const hrrt_ctx *ctx, const char *image_path, const hrrt_image_info *info) {
+    const hrrt::Geometry &G = ctx->geo;
+    std::string name(base_name(image_path)), ext;
+    const size_t dot = name.rfind('.');
+    if (dot != std::string::npos) {
+        ext = name.substr(dot);
+        name.resize(dot);
+    }
+    const std::string hdr = std::string(image_path) + ".hdr";
+    FILE *out = std::fopen(hdr.c_str(), "wt");
+    if (!out) return fail(HRRT_ERR_ARG, "cannot open image header file '%s'", hdr.c_str());
+    FILE *sino = nullptr;      // lines of the sinogram's header are carried forward
+    const char *sino_file = (info && info->true_file && *info->true_file) ? info->true_file
+                            : (info && info->prompt_file && *info->prompt_file) ? info->prompt_file
+                                                                                : nullptr;
+    if (sino_file) sino = std::fopen((std::string(sino_file) + ".hdr").c_str(), "r");
+    std::fprintf(out, "!INTERFILE\n");
+    std::fprintf(out, "name of data file := %s%s\n", name.c_str(), ext.c_str());
+    std::fprintf(out, "data format := image\n");
+    std::fprintf(out, "number format := %s\n", "float");
+    std::fprintf(out, "number of bytes per pixel := %d\n", 4);
+    std::fprintf(out, "number of dimensions := 3\n");
+    std::fprintf(out, "matrix size [1] := %d\n", G.N);
+    std::fprintf(out, "matrix size [2] := %d\n", G.N);
+    std::fprintf(out, "matrix size [3] := %d\n", G.z_pixels);
+    std::fprintf(out, "scaling factor (mm/pixel) [1] := %f\n", G.x_size);
+    std::fprintf(out, "scaling factor (mm/pixel) [2] := %f\n", G.y_size);
+    std::fprintf(out, "scaling factor (mm/pixel) [3] := %f\n", G.z_size);
+    static const char *methods[] = {"UW", "AW", "ANW", "OP"};
+    const int rt = info ? info->recontype : 3;
+    if (rt >= 0 && rt <= 3)
+        std::fprintf(out, "reconstruction method := inki-2006-may-OSEM3D-%s%s\n", methods[rt], ctx->psf_on ? "-PSF" : "");
+    std::fprintf(out, "reconstruction version := %s\n", (info && info->sw_version) ? info->sw_version : "");
+    std::fprintf(out, "reconstruction build ID := %s\n", (info && info->sw_build_id) ? info->sw_build_id : "");
+    if (info) {
+        const struct { const char *key, *file; } used[] = {
+            {"true", info->true_file},   {"prompt", info->prompt_file}, {"delayed", info->delayed_file},
+            {"norm", info->norm_file},   {"atten", info->atten_file},   {"scatter", info->scatter_file}};
+        for (const auto &u : used) {
+            if (!u.file || !*u.file) continue;
+            std::fprintf(out, "%s file used := %s\n", u.key, base_name(u.file));
+            if (u.file == info->scatter_file) {
+                const float scf = scatter_fraction(u.file);
+                if (scf > 0) std::fprintf(out, "scatter fraction := %g\n", scf);
+            }
+        }
+    }
+    std::fprintf(out, "number of iterations := %d\n", info ? info->iterations : 0);
+    std::fprintf(out, "number of subsets := %d\n", G.p.subsets);
+    if (sino) {
+        static const char *skip[] = {"name of data file",    "data format",       "number format",
+                                     "number of bytes per",  "number of dimensions", "matrix size",
+                                     "scaling factor",       "reconstruction method", "true file used",
+                                     "prompt file used",     "delayed file used", "norm file used",
+                                     "atten file used",      "scatter file used", "number of iterations",
+                                     "number of subsets",    "CPSCC-OSEM3D build", "CPSCC-Server build"};
+        char line[1024];
+        while (std::fgets(line, sizeof line, sino)) {
+            if (!std::strncmp(line, "!INTERFILE", 10) || !std::strncmp(line, "!Interfile", 10) ||
+                !std::strncmp(line, "!interfile", 10))
+                continue;
+            bool drop = false;
+            for (const char *k : skip) drop = drop || std::strstr(line, k) != nullptr;
+            if (!drop) std::fprintf(out, "%s", line);
+        }
+        std::fclose(sino);
+    }
+    std::fclose(out);
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_write_image(hrrt_ctx *ctx, const float *d_image, const char *path, int flip,
+                                const hrrt_image_info *info, void *stream) {
+    if (!ctx || !d_image || !path) return fail(HRRT_ERR_ARG, "null argument");
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->flat, sizeof(float) * flat_floats(ctx)))) return r;
+    std::vector<float> host(flat_floats(ctx));
+    HIP_TRY(hrrt::launch_flat_image(ctx->dg, d_image, ctx->flat, 1, flip ? 1 : 0, S(stream)));
+    HIP_TRY(hipMemcpyAsync(host.data(), ctx->flat, sizeof(float) * host.size(), hipMemcpyDeviceToHost, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(HRRT_ERR_ARG, "cannot open '%s'", path);
+    const size_t plane = (size_t)ctx->geo.N * ctx->geo.N;
+    for (int z = 0; z < ctx->geo.z_pixels; z++)     // plane by plane, as write_flat_image does
+        if (std::fwrite(host.data() + z * plane, sizeof(float), plane, f) != plane) {
+            std::fclose(f);
+            return fail(HRRT_ERR_ARG, "write error on '%s'", path);
+        }
+    std::fclose(f);
+    return write_header(ctx, path, info);
+}
+
+extern "C" int hrrt_read_image(hrrt_ctx *ctx, const char *path, float *d_image, int flip, void *stream) {
+    if (!ctx || !d_image || !path) return fail(HRRT_ERR_ARG, "null argument");
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->flat, sizeof(float) * flat_floats(ctx)))) return r;
+    std::vector<float> host(flat_floats(ctx));
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(HRRT_ERR_ARG, "cannot open initial image '%s'", path);
+    const size_t got = std::fread(host.data(), sizeof(float), host.size(), f);
+    std::fclose(f);
+    if (got != host.size()) return fail(HRRT_ERR_ARG, "'%s' holds %zu floats, %zu expected", path, got, host.size());
+    HIP_TRY(hipMemcpyAsync(ctx->flat, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice, S(stream)));
+    HIP_TRY(hrrt::launch_flat_image(ctx->dg, ctx->flat, d_image, 0, flip ? 1 : 0, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_write_normfac(hrrt_ctx *ctx, const float *d_normfac, const char *path, void *stream) {
+    if (!ctx || !d_normfac || !path) return fail(HRRT_ERR_ARG, "null argument");
+    const size_t n = ctx->vol_floats();
+    std::vector<float> host(n);
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(HRRT_ERR_ARG, "cannot open '%s'", path);
+    for (int s = 0; s < ctx->geo.p.subsets; s++) {   // subset after subset, [X][Y][Z4] each
+        if (hipMemcpyAsync(host.data(), d_normfac + s * n, sizeof(float) * n, hipMemcpyDeviceToHost, S(stream)) !=
+                hipSuccess ||
+            hipStreamSynchronize(S(stream)) != hipSuccess) {
+            std::fclose(f);
+            return fail(HRRT_ERR_HIP, "copy of sensitivity subset %d failed", s);
+        }
+        if (std::fwrite(host.data(), sizeof(float), n, f) != n) {
+            std::fclose(f);
+            return fail(HRRT_ERR_ARG, "write error on '%s'", path);
+        }
+    }
+    std::fclose(f);
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_read_normfac(hrrt_ctx *ctx, const char *path, float *d_normfac, void *stream) {
+    if (!ctx || !d_normfac || !path) return fail(HRRT_ERR_ARG, "null argument");
+    const size_t n = ctx->vol_floats();
+    std::vector<float> host(n);
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(HRRT_ERR_ARG, "cannot open '%s'", path);
+    for (int s = 0; s < ctx->geo.p.subsets; s++) {
+        if (std::fread(host.data(), sizeof(float), n, f) != n) {
+            std::fclose(f);
+            return fail(HRRT_ERR_ARG, "'%s' ends before subset %d", path, s);
+        }
+        if (hipMemcpyAsync(d_normfac + s * n, host.data(), sizeof(float) * n, hipMemcpyHostToDevice, S(stream)) !=
+                hipSuccess ||
+            hipStreamSynchronize(S(stream)) != hipSuccess) {
+            std::fclose(f);
+            return fail(HRRT_ERR_HIP, "copy of sensitivity subset %d failed", s);
+        }
+    }
+    std::fclose(f);
     return HRRT_OK;
 }
 

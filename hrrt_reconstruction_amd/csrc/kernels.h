@@ -98,6 +98,8 @@ hipError_t launch_estimate_to_paircol(const DevGeom &g, float *prj, float *pairc
 hipError_t launch_ratio_w3(const DevGeom &g, float *est, const float *add, const int16_t *prompt, hipStream_t s);
 hipError_t launch_update_image(const DevGeom &g, float *image, float *corr, const float *nf, hipStream_t s);
 hipError_t launch_normfac_invert(const DevGeom &g, const float *sens, float *out, hipStream_t s);
+// image [X][Y][Z4] <-> flat planes [Z][N][N] (to_flat = 1: image -> flat); flip swaps x and y
+hipError_t launch_flat_image(const DevGeom &g, const float *src, float *dst, int to_flat, int flip, hipStream_t s);
 // rotation of the batch's pairs summed into corr (FOV voxels), no update
 hipError_t launch_rotate_sum(const DevGeom &g, const PairBatch &b, const float *bufA, const float *bufB, float *corr,
                              hipStream_t s);

@@ -854,6 +854,53 @@ __global__ void __launch_bounds__(256) psf_yz_kernel(const DevGeom g, const floa
     }
 }
 
+// ---------------------------------------------------------------------------
+// Flat image files, planes [z][v][u] (SaveImage S3:4023-4035 with write_flat_image
+// FIO:300-333; the -I read of Init_Image S3:3970-3993).  Image voxel (a, b) holds flat
+// (v, u) = (b, a), or (a, b) when flipped; only FOV voxels are copied, the rest is 0.
+// A workgroup transposes a 64 (u) x 64 (z) tile of one v through LDS, so both sides
+// move whole 256-byte rows.
+// ---------------------------------------------------------------------------
+template <bool TO_FLAT>
+__global__ void __launch_bounds__(256) flat_image_kernel(const DevGeom g, const float *__restrict__ src,
+                                                         float *__restrict__ dst, int flip) {
+    __shared__ float tile[64][65];
+    const int v = blockIdx.y, u0 = blockIdx.x * 64, z0 = blockIdx.z * 64, N = g.N;
+    const int t = threadIdx.x & 63, w = threadIdx.x >> 6;
+    auto voxel = [&](int u, int &a, int &b) {
+        a = flip ? v : u;
+        b = flip ? u : v;
+        return u < N && b >= g.cyl_lo[a] && b < g.cyl_hi[a];
+    };
+    if (TO_FLAT) {
+        for (int i = w; i < 64; i += 4) {            // image rows: u0 + i, planes z0 + t
+            int a, b;
+            const int z = z0 + t;
+            float val = 0.0f;
+            if (voxel(u0 + i, a, b) && z < g.Z) val = src[((size_t)a * N + b) * g.Z4 + z];
+            tile[i][t] = val;
+        }
+        __syncthreads();
+        for (int i = w; i < 64; i += 4) {            // flat rows: plane z0 + i, columns u0 + t
+            const int z = z0 + i, u = u0 + t;
+            if (z < g.Z && u < N) dst[((size_t)z * N + v) * N + u] = tile[t][i];
+        }
+    } else {
+        for (int i = w; i < 64; i += 4) {            // flat rows: plane z0 + i, columns u0 + t
+            const int z = z0 + i, u = u0 + t;
+            tile[t][i] = (z < g.Z && u < N) ? src[((size_t)z * N + v) * N + u] : 0.0f;
+        }
+        __syncthreads();
+        for (int i = w; i < 64; i += 4) {            // image rows: u0 + i, planes z0 + t
+            int a, b;
+            const int z = z0 + t;
+            if (u0 + i >= N || z >= g.Z4) continue;
+            const bool in = voxel(u0 + i, a, b);       // (a, b) set either way; every voxel is written
+            dst[((size_t)a * N + b) * g.Z4 + z] = (in && z < g.Z) ? tile[i][t] : 0.0f;
+        }
+    }
+}
+
 __device__ __forceinline__ float invert1(float s, float eps1) { return fabsf(s) > eps1 ? (1.0f / s) : 0.0f; }
 
 template <bool KAHAN>
@@ -1357,6 +1404,13 @@ hipError_t launch_convolve3d(const DevGeom &g, const float *in, float *out, floa
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(psf_yz_kernel, rot_grid(g), dim3(64, 4), 0, s, g, reinterpret_cast<const float4 *>(buf), out,
                        taps_y, hy, normz, z_min, z_max);
+    return hipGetLastError();
+}
+
+hipError_t launch_flat_image(const DevGeom &g, const float *src, float *dst, int to_flat, int flip, hipStream_t s) {
+    dim3 grid((g.N + 63) / 64, g.N, (g.Z4 + 63) / 64), block(256);
+    if (to_flat) hipLaunchKernelGGL(flat_image_kernel<true>, grid, block, 0, s, g, src, dst, flip);
+    else hipLaunchKernelGGL(flat_image_kernel<false>, grid, block, 0, s, g, src, dst, flip);
     return hipGetLastError();
 }
 

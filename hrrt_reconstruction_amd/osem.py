@@ -146,6 +146,40 @@ class Osem3D(Layout):
         _lib.call("hrrt_calculate_normfac", self.ctx, _p(invnorm), _p(nf), _stream())
         return nf
 
+    # ---- image and sensitivity files (SaveImage S3:4016, Init_Image S3:3970, FIO:300-427) ----
+    def image_to_flat(self, image, flip: bool = False):
+        flat = torch.zeros((self.Z, self.N, self.N), dtype=torch.float32, device=image.device)
+        _lib.call("hrrt_image_to_flat", self.ctx, _p(image), _p(flat), int(flip), _stream())
+        return flat
+
+    def image_from_flat(self, flat, flip: bool = False):
+        image = torch.empty((self.N, self.N, self.Z4), dtype=torch.float32, device=flat.device)
+        _lib.call("hrrt_image_from_flat", self.ctx, _p(flat), _p(image), int(flip), _stream())
+        return image
+
+    def write_image(self, image, path: str, flip: bool = False, iterations: int = 0, recontype: int = 3,
+                    sw_version: str = "", sw_build_id: str = "", **files):
+        """Flat image file plus its Interfile header; files: true_file, prompt_file, delayed_file,
+        norm_file, atten_file, scatter_file (paths, for the header)."""
+        enc = {k: (v.encode() if v else None) for k, v in files.items()}
+        info = _lib.ImageInfo(recontype=recontype, iterations=iterations, sw_version=sw_version.encode(),
+                              sw_build_id=sw_build_id.encode(), **enc)
+        _lib.call("hrrt_write_image", self.ctx, _p(image), str(path).encode(), int(flip), C.byref(info),
+                  _stream())
+
+    def read_image(self, path: str, flip: bool = False):
+        image = torch.empty((self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)
+        _lib.call("hrrt_read_image", self.ctx, str(path).encode(), _p(image), int(flip), _stream())
+        return image
+
+    def write_normfac(self, normfac, path: str):
+        _lib.call("hrrt_write_normfac", self.ctx, _p(normfac), str(path).encode(), _stream())
+
+    def read_normfac(self, path: str):
+        nf = torch.zeros((self.subsets, self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)
+        _lib.call("hrrt_read_normfac", self.ctx, str(path).encode(), _p(nf), _stream())
+        return nf
+
     # ---- OSEM (CalculateOsem3d S3:4098) ----
     def init_image(self):
         img = torch.empty((self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)

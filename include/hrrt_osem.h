@@ -254,6 +254,32 @@ int hrrt_get_psf_kernel(const hrrt_ctx *ctx, int axis, float *taps, int *size);
 /* convolve3d (psf.cpp:441-476) of an image [X][Y][Z4]; d_out may be d_in.  Only voxels
  * inside the FOV are written. */
 int hrrt_convolve3d(hrrt_ctx *ctx, const float *d_in, float *d_out, void *stream);
+/* ---------------- image and sensitivity files ----------------
+ * Flat image files hold planes [z_pixels][N][N] float32, x fastest (SaveImage S3:4023-4035,
+ * write_flat_image FIO:300-333).  flip = the reference's output/input flip (x <-> y).  Only
+ * FOV voxels are copied; the image side outside the FOV and the pad plane become 0. */
+int hrrt_image_to_flat(hrrt_ctx *ctx, const float *d_image, float *d_flat, int flip, void *stream);
+/* Init_Image with -I (S3:3970-3993): the inverse of hrrt_image_to_flat. */
+int hrrt_image_from_flat(hrrt_ctx *ctx, const float *d_flat, float *d_image, int flip, void *stream);
+/* Header contents of write_image_header (write_image_header.cpp:67-319); NULL strings are
+ * "not used".  recontype: 0 UW, 1 AW, 2 ANW, 3 OP (the -W weighting). */
+typedef struct {
+    int recontype;
+    int iterations;
+    const char *true_file, *prompt_file, *delayed_file, *norm_file, *atten_file, *scatter_file;
+    const char *sw_version, *sw_build_id;
+} hrrt_image_info;
+/* SaveImage (S3:4016-4096): writes `path` (flat planes) and `path`.hdr (Interfile), carrying
+ * forward the lines of the true or prompt file's .hdr; synchronises `stream`. */
+int hrrt_write_image(hrrt_ctx *ctx, const float *d_image, const char *path, int flip, const hrrt_image_info *info,
+                     void *stream);
+/* Reads an initial image file (-I) into d_image; synchronises `stream`. */
+int hrrt_read_image(hrrt_ctx *ctx, const char *path, float *d_image, int flip, void *stream);
+/* The sensitivity cache normfac.i: subsets x [X][Y][Z4] float32 (write_norm / read_norm
+ * FIO:360-427); synchronise `stream`. */
+int hrrt_write_normfac(hrrt_ctx *ctx, const float *d_normfac, const char *path, void *stream);
+int hrrt_read_normfac(hrrt_ctx *ctx, const char *path, float *d_normfac, void *stream);
+
 /* update_estimate_w3 (S3:2474): est = prompt==0 ? 0 : (est+add > eps3 ? prompt/(est+add) : 1),
  * over the [2S][N] rows of one view pair. */
 int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float *d_add, const int16_t *d_prompt, void *stream);

@@ -162,6 +162,10 @@ void     og_osem_subset_psf(const og_geom *g, const og_psf *p, float *image, flo
                             const float *normfac_subset, const float *add, const int16_t *prompt, int isubset,
                             int nthreads, int workers);
 
+/* ---- flat image files: planes [z_pixels][N][N] ---- */
+void     og_image_to_flat(const og_geom *g, const float *image, float *flat, int flip);    /* SaveImage S3:4023 */
+void     og_image_from_flat(const og_geom *g, const float *flat, float *image, int flip);  /* Init_Image S3:3970 */
+
 /* CalculateNormfac S3:3760: normfac_all [subsets][X][Y][Z4] receives 1/sens per subset.
  * invnorm: [S][views][N] (= 1/(N*A)); it is rescaled in place, as the reference does.
  * nthreads selects the reference's split between the *_thread1 and *2 variants

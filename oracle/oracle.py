@@ -104,6 +104,8 @@ def lib():
         L.og_gather_pair_s.argtypes = [G, vp, C.c_int, vp]
         L.og_calculate_normfac.argtypes = [G, vp, vp, C.c_int]
         L.og_osem_subset.argtypes = [G, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+        L.og_image_to_flat.argtypes = [G, vp, vp, C.c_int]
+        L.og_image_from_flat.argtypes = [G, vp, vp, C.c_int]
         L.og_psf_create.restype = vp
         L.og_psf_create.argtypes = [G, C.c_float, C.c_float, C.c_float]
         L.og_psf_destroy.argtypes = [vp]
@@ -359,6 +361,17 @@ class Oracle:
     def osem_subset(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
         self.L.og_osem_subset(self.gp, _ptr(image), _ptr(correction), _ptr(normfac_subset), _ptr(add),
                               _ptr(prompt), isubset, nthreads, workers)
+
+    # ---- flat image files ----
+    def image_to_flat(self, image, flip=False):
+        flat = np.zeros((self.Z, self.N, self.N), np.float32)
+        self.L.og_image_to_flat(self.gp, _ptr(np.ascontiguousarray(image, np.float32)), _ptr(flat), int(flip))
+        return flat
+
+    def image_from_flat(self, flat, flip=False):
+        image = self.image_zeros()
+        self.L.og_image_from_flat(self.gp, _ptr(np.ascontiguousarray(flat, np.float32)), _ptr(image), int(flip))
+        return image
 
     # ---- PSF resolution modelling (-B) ----
     def set_psf(self, fwhm1=0.0, fwhm2=0.0, ratio=0.0):

@@ -780,3 +780,33 @@ void og_osem_subset_psf(const og_geom *g, const og_psf *p, float *image, float *
     free(buf);
     free(image_psf);
 }
+
+/* ------------------------------------------------------------------ */
+/* Flat image files                                                    */
+/* ------------------------------------------------------------------ */
+void og_image_to_flat(const og_geom *g, const float *image, float *flat, int flip)
+{   /* SaveImage S3:4023-4035: tmpimage[z][y][x] = image[x][y][z] (image[y][x][z] flipped), FOV only */
+    const int N = g->N, Z4 = g->z4;
+    int x, y, z;
+    memset(flat, 0, sizeof(float) * (size_t)g->z_pixels * N * N);
+    for (x = 0; x < N; x++)
+        for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++)
+            for (z = 0; z < g->z_pixels; z++)
+                flat[((size_t)z * N + y) * N + x] = flip ? image[((size_t)y * N + x) * Z4 + z]
+                                                         : image[((size_t)x * N + y) * Z4 + z];
+}
+
+void og_image_from_flat(const og_geom *g, const float *flat, float *image, int flip)
+{   /* Init_Image -I S3:3970-3993: tmp[x][y] = plane z; image[y][x][z] = tmp[x][y] (image[x][y][z]
+     * flipped), FOV only; the rest of the image is 0 here (the reference leaves it unset) */
+    const int N = g->N, Z4 = g->z4;
+    int x, y, z;
+    memset(image, 0, sizeof(float) * (size_t)N * N * Z4);
+    for (z = 0; z < g->z_pixels; z++)
+        for (x = 0; x < N; x++)
+            for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++) {
+                const float v = flat[((size_t)z * N + x) * N + y];
+                if (flip) image[((size_t)x * N + y) * Z4 + z] = v;
+                else image[((size_t)y * N + x) * Z4 + z] = v;
+            }
+}
