@@ -296,6 +296,7 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
     // repeats the tables' float operations exactly.  Rows below ymin are skipped
     // (PJ:1920-1921); the skip rule is checked to hold for exactly the rows below a threshold.
     std::vector<int4> fplanes;
+    int fp_fast = 1;   // positions never negative: the forward projector may take fract for zz - (int)zz
     for (int yr = 0; yr < Z; yr++) fplanes.push_back(make_int4(yr << 8, 0, 0, 0));
     for (int k = 1; k <= G.K; k++) {
         const int th = 2 * k, yb = G.yr_bottom[th], yt = G.yr_top[th], ym = yb + ZS;
@@ -317,6 +318,7 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
             for (int y = ymin; y < c; y++) {
                 const float zz1 = bf + (y - c) * t, zz2 = bf + (c - y) * t;
                 const int z1 = (int)zz1, z2 = (int)zz2;
+                if (zz1 < 0.0f || zz2 < 0.0f) fp_fast = 0;
                 int rz, rz2;
                 float rf1, rf2;
                 const size_t li = (size_t)y * GM + k - 1;
@@ -515,6 +517,7 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         return r;
     }
     d.fp_np = (int)fplanes.size();
+    d.fp_fast = fp_fast;
     d.fp_maxt = 4;
     d.fp_nsplit = 0;
     d.bp_waves = 16;

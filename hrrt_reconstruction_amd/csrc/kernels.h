@@ -28,6 +28,7 @@ struct DevGeom {
     const int4 *fp_planes;               // [fp_np] forward-projector planes, axial first, then by (k, yr):
                                          //  {k | yr << 8 | delta << 16, bits(base plane), bits(tan), ymin}
     int fp_np;
+    int fp_fast;                         // 1: every forward position of a row the kernel projects is >= 0
     const int16_t *pseg;                 // [spp] padded plane -> segment (-1 on guards)
     const int16_t *plane_seg;            // [S] internal plane -> segment
     const int *bp_src;                   // [bp_lds + 1] LDS entry -> pair-column plane (-1 = zero)

@@ -8,6 +8,8 @@
 // FIO = hrrt_osem3d/file_io_processor.cpp.
 #include "kernels.h"
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace hrrt {
@@ -222,12 +224,17 @@ __device__ __forceinline__ void fp_write1(const DevGeom &g, float4 acc, int seg,
     }
 }
 
-template <int MAXT, int ITEMS, int MODE>
+// FAST (HRRT's 207 planes, positions checked non-negative on the host): the staged
+// column length is a compile-time constant, so every LDS read of a row takes its buffer
+// and neighbour as immediate offsets, and the axial fraction is one v_fract.
+constexpr int FP_PL_FAST = 216;
+
+template <int MAXT, int ITEMS, int MODE, bool FAST>
 __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBatch b,
                                                    const float4 *__restrict__ image, float *__restrict__ out,
                                                    FpIO io, int nsplit) {
     extern __shared__ float4 lds[];
-    const int PL = g.Z4 + 8;   // staged planes per column (zero beyond z_pixels)
+    const int PL = FAST ? FP_PL_FAST : g.Z4 + 8;   // staged planes per column (zero beyond z_pixels)
     float4 *rimc = lds;
     float4 *stage = lds + PL;  // [2][2][PL]
     // longest rays first (x = N/2 has the most y steps), then plane share, then view pair
@@ -288,12 +295,13 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
         acc2[t] = v;
     }
 
-    for (int y = ystart; y < c; y++) {
-        const int st = (y - ystart) & 1;
+    // one row: project the staged row y (buffer ST) while the next row's samples load
+    auto row = [&](auto stc, int y) {
+        constexpr int st = decltype(stc)::value;
         SampleRegs<ITEMS> nxt;
         if (y + 1 < c) sample8_load<ITEMS>(g, image, cosv, sinv, x, y + 1, nxt);
         const float4 *pa = stage + st * 2 * PL, *pb = pa + PL;
-        const float fy1 = (float)(y - c), fy2 = (float)(c - y);
+        const float fy2 = (float)(c - y);   // (y - c) * tan = -((c - y) * tan) exactly
 #pragma unroll
         for (int t = 0; t < MAXT; t++) {
             if (code[t] < 0) continue;
@@ -305,9 +313,17 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
             }
             if (y < ymin[t]) continue;   // PJ:1920-1921
             const int delta = code[t] >> 16;
-            const float zz1 = pbase[t] + fy1 * ptan[t], zz2 = pbase[t] + fy2 * ptan[t];
+            const float pt = fy2 * ptan[t];
+            const float zz1 = pbase[t] - pt, zz2 = pbase[t] + pt;
             const int i1 = (int)zz1, i2 = (int)zz2;
-            const float f1 = zz1 - i1, f2 = zz2 - i2;
+            float f1, f2;
+            if (FAST) {        // zz >= 0: zz - trunc(zz) = zz - floor(zz)
+                f1 = __builtin_amdgcn_fractf(zz1);
+                f2 = __builtin_amdgcn_fractf(zz2);
+            } else {
+                f1 = zz1 - i1;
+                f2 = zz2 - i2;
+            }
             const int z = i1 + delta, z2 = i2 + delta;
             const float4 a0 = pa[z], a1 = pa[z + 1], b0 = pb[z], b1 = pb[z + 1];
             const float4 d0 = pb[z2], d1 = pb[z2 + 1], e0 = pa[z2], e1 = pa[z2 + 1];
@@ -317,6 +333,10 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
         }
         if (y + 1 < c) sample8_store<ITEMS>(g, nxt, stage + (st ^ 1) * 2 * PL, stage + (st ^ 1) * 2 * PL + PL);
         __syncthreads();
+    };
+    for (int y = ystart; y < c; y += 2) {   // row ystart + 2j reads buffer 0, the next one buffer 1
+        row(std::integral_constant<int, 0>{}, y);
+        if (y + 1 < c) row(std::integral_constant<int, 1>{}, y + 1);
     }
 
     // scaling PJ:1953-1975 and write-out
@@ -1238,18 +1258,18 @@ __global__ void __launch_bounds__(256) moments_reduce_kernel(const double *__res
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int MAXT, int ITEMS>
+template <int MAXT, int ITEMS, bool FAST>
 static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float *image, float *out, const FpIO &io,
                               int mode, int nsplit, int threads, hipStream_t s) {
     dim3 grid(g.c * nsplit * b.npairs * b.nframes), block(threads);
-    size_t shm = sizeof(float4) * 5 * (g.Z4 + 8);
+    size_t shm = sizeof(float4) * 5 * (FAST ? FP_PL_FAST : g.Z4 + 8);
     const float4 *im = reinterpret_cast<const float4 *>(image);
     if (mode == FP_RATIO)
-        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO>), grid, block, shm, s, g, b, im, out, io, nsplit);
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO, FAST>), grid, block, shm, s, g, b, im, out, io, nsplit);
     else if (mode == FP_RATIO16)
-        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO16>), grid, block, shm, s, g, b, im, out, io, nsplit);
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO16, FAST>), grid, block, shm, s, g, b, im, out, io, nsplit);
     else
-        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_PLAIN>), grid, block, shm, s, g, b, im, out, io, nsplit);
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_PLAIN, FAST>), grid, block, shm, s, g, b, im, out, io, nsplit);
     return hipGetLastError();
 }
 
@@ -1279,8 +1299,10 @@ hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *ima
     forward_shape(g, maxt, nsplit, threads, it2);
 #define FP_CASE(M)                                                                                    \
     case M:                                                                                           \
-        return it2 <= 1 ? fp_launch_t<M, 1>(g, b, image, out, io, mode, nsplit, threads, s)           \
-                        : fp_launch_t<M, 2>(g, b, image, out, io, mode, nsplit, threads, s);
+        return it2 <= 1 ? fp_launch_t<M, 1, false>(g, b, image, out, io, mode, nsplit, threads, s)    \
+                        : fp_launch_t<M, 2, false>(g, b, image, out, io, mode, nsplit, threads, s);
+    if (maxt == 4 && it2 <= 1 && g.fp_fast && g.Z4 + 8 == FP_PL_FAST)
+        return fp_launch_t<4, 1, true>(g, b, image, out, io, mode, nsplit, threads, s);
     switch (maxt) {
         FP_CASE(1)
         FP_CASE(2)
