@@ -51,6 +51,14 @@ class ImageInfo(C.Structure):
     _fields_ = [("recontype", C.c_int), ("iterations", C.c_int)] + [
         (n, C.c_char_p) for n in ("true_file", "prompt_file", "delayed_file", "norm_file", "atten_file",
                                   "scatter_file", "sw_version", "sw_build_id")]
+
+
+class DelaysParams(C.Structure):
+    """hrrt_delays_params (gen_delays options)."""
+    _fields_ = [("span", C.c_int), ("maxrd", C.c_int), ("lr_type", C.c_int), ("pitch", C.c_float),
+                ("diam", C.c_float), ("thick", C.c_float), ("tau", C.c_float), ("lut_file", C.c_char_p)]
+
+
 SIGNATURES = {
     "hrrt_create": (_I, [C.POINTER(Params), C.POINTER(_VP)]),
     "hrrt_destroy": (None, [_VP]),
@@ -97,6 +105,16 @@ SIGNATURES = {
     "hrrt_normfac_calculate": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_forward_proj3d_host": (_I, [_VP, _VP, _VP, _I]),
     "hrrt_back_proj3d_host": (_I, [_VP, _VP, _VP, _I]),
+    # include/hrrt_randoms.h
+    "hrrt_delays_create": (_I, [C.POINTER(DelaysParams), C.POINTER(_VP)]),
+    "hrrt_delays_destroy": (None, [_VP]),
+    "hrrt_delays_get_dims": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "hrrt_delays_get_lut": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "hrrt_delays_save_lut": (_I, [_VP, C.c_char_p]),
+    "hrrt_delays_singles": (_I, [_VP, _VP, C.c_float, _VP, _VP]),
+    "hrrt_delays_smooth": (_I, [_VP, _VP, C.c_float, _VP, _VP]),
+    "hrrt_gen_delays": (_I, [_VP, _VP, C.c_float, _VP, _VP, _VP, _VP]),
+    "hrrt_delays_write": (_I, [_VP, _VP, C.c_char_p, _VP]),
 }
 
 _lib = None

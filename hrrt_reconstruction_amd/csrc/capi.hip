@@ -200,6 +200,11 @@ static inline int bits(float f) {
 
 extern "C" const char *hrrt_last_error(void) { return g_err.c_str(); }
 
+int hrrt::set_last_error(int code, const char *msg) {
+    g_err = msg;
+    return code;
+}
+
 extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
     if (!params || !out) return fail(HRRT_ERR_ARG, "null argument");
     *out = nullptr;

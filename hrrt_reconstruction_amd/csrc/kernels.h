@@ -114,4 +114,7 @@ hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *
                         const float *scatter_f, const float *atten_f, int16_t *prompt, float *add, float *invnorm,
                         uint8_t *gapmask, int *colmask, hipStream_t s);
 
+// Records msg as hrrt_last_error()'s message of this thread and returns code.
+int set_last_error(int code, const char *msg);
+
 }  // namespace hrrt

@@ -1,0 +1,667 @@
+// randoms.hip -- smoothed randoms from crystal singles (gen_delays_lib) on gfx950, and its
+// C ABI (include/hrrt_randoms.h).
+//
+// Host side (once per context): HRRT crystal geometry, segment tables, the module-pair LUT
+// (each transaxial crystal pair of 20 head pairs -> sinogram bin, 1/(d tan), closest-approach
+// z), the LUT grouped by sinogram bin, and the singles iteration.
+// Device side (per smoothing):
+//   sings_prepare_kernel    singles [layer][ring][head][x] -> [layer][head][x][ring], so that a
+//                           crystal's 104 rings are contiguous, and their largest magnitude
+//   delays_column_kernel    one workgroup per sinogram bin: every crystal pair of every LOR that
+//                           lands in the bin, accumulated into the bin's column of planes in
+//                           LDS (64-bit fixed point, ds_add_u64), then stored contiguously
+//   column_transpose_kernel [bin][plane] columns -> [plane][view][bin] output, 64 x 64 tiles
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hrrt_randoms.h"
+#include "kernels.h"
+
+namespace {
+
+constexpr int NHEADS = 8, NDOIS = 2, NX = 72, NY = 104;
+constexpr int NXD = NX * NDOIS;                   // 144 crystals across a head, both layers
+constexpr int NCRYS = NHEADS * NDOIS * NX * NY;   // 119808
+constexpr int XYH = NX * NY * NHEADS;             // one layer, [ring][head][x]
+constexpr int HX = NHEADS * NX;                   // one ring over the heads
+constexpr int NMP = 20;
+constexpr int SEGMAP = 63;                        // segment rows of the plane map (lor_sinogram_map.cpp:154-158)
+
+// HRRT_MPAIRS geometry_info.h:68-73
+constexpr int kMpairs[NMP + 1][2] = {{-1, -1}, {0, 2}, {0, 3}, {0, 4}, {0, 5}, {0, 6}, {1, 3}, {1, 4},
+                                     {1, 5},   {1, 6}, {1, 7}, {2, 4}, {2, 5}, {2, 6}, {2, 7}, {3, 5},
+                                     {3, 6},   {3, 7}, {4, 6}, {4, 7}, {5, 7}};
+// span-3 segment -> span-9 segment (conversiontable, segment_info_fromarchive.cpp:23-26)
+constexpr int kS3toS9[45] = {0, 0, 0, 1, 2, 1, 2, 1, 2, 3, 4, 3, 4, 3, 4, 5, 6, 5, 6, 5, 6, 7, 8,
+                             7, 8, 7, 8, 9, 10, 9, 10, 9, 10, 11, 12, 11, 12, 11, 12, 13, 14, 13, 14, 13, 14};
+
+struct Sol {       // SOL lor_sinogram_map.h:18-22 (also the LUT file record)
+    int nsino;
+    float d, z;
+};
+static_assert(sizeof(Sol) == 12, "LUT record is 12 bytes");
+
+struct Tables {    // device tables of one context
+    const int4 *lor;      // [nlor] {A ring-column offset, B ring-column offset, bits(d), bits(z)}
+    const int *lor_off;   // [nsino + 1] LORs of bin b: lor[lor_off[b] .. lor_off[b+1])
+    const int4 *seg;      // [64] {first plane, last plane, file offset, 0} per segment code
+    const float2 *z;      // [104] {ring z (cm), ring z in planes + 0.5}
+    int nsino, nplanes, window;
+};
+
+// ---------------------------------------------------------------- kernels
+
+// One workgroup: singles [layer][ring][head][x] -> [layer][head][x][ring] (a crystal's 104 rings
+// contiguous) and the largest |singles| into out[NCRYS], which sets the fixed-point scale.
+constexpr int PREP_THREADS = 1024;
+
+__global__ __launch_bounds__(PREP_THREADS) void sings_prepare_kernel(const float *__restrict__ in,
+                                                                     float *__restrict__ out) {
+    __shared__ float wmax[PREP_THREADS / 64];
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < NCRYS; i += PREP_THREADS) {
+        const int ring = i % NY, x = (i / NY) % NX, head = (i / (NY * NX)) % NHEADS, layer = i / (NY * NX * NHEADS);
+        const float v = in[layer * XYH + ring * HX + head * NX + x];
+        out[i] = v;
+        m = fmaxf(m, fabsf(v));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < PREP_THREADS / 64; w++) m = fmaxf(m, wmax[w]);
+        out[NCRYS] = m;
+    }
+}
+
+// Exponent k of the fixed-point unit 2^-k: every term s_A*tau*T*s_B is at most
+// M = smax^2*tau*T < 2^E, so a term scaled by 2^k = 2^(36-E) is below 2^36 and a sum of up
+// to 2^27 terms stays inside int64.
+__device__ inline int fixed_point_exponent(float smax, float tauftime) {
+    const float m = smax * smax * tauftime;
+    int e = 0;
+    if (m > 0.0f && m <= 3.0e38f) frexpf(m, &e);
+    return min(36 - e, 127);
+}
+
+// Term -> int64 in units of 2^-k, rounded to nearest: x = v*2^k is exact (a power of two),
+// and so are its split into x = hi*2^32 + lo (hi integral) and lo's rounding below 2^32.
+__device__ inline unsigned long long to_fixed(float v, float up) {
+    const float x = v * up;
+    const float hi = floorf(x * 0x1p-32f);
+    const float lo = x - hi * 0x1p32f;
+    return ((unsigned long long)(long long)(int)hi << 32) + (unsigned long long)(unsigned)rintf(lo);
+}
+
+// One sinogram bin per workgroup of 512 threads (8 waves).  A wave covers 64 B rings (wave
+// parity: rings 0..63 or 64..103) for every fourth A ring (row = wave / 2).  The A singles
+// and ring z of the LOR sit one per lane in registers and are read with v_readlane for the
+// wave-uniform A ring, so the ring loop issues no memory instruction but the segment-table
+// read and the LDS add.  Each crystal pair adds s_A*tau*T*s_B to plane segplane(code, plane)
+// of the bin's column, with the reference's float operations (compute_delays
+// gen_delays_fromarchive.cpp:205-222):
+//   plane = (int)(cay + z*dz),  seg = 0.5 + dz*d,  code = seg < 0 ? 1 - 2(int)seg : 2(int)seg
+// The column accumulates in 64-bit fixed point (ds_add_u64; LDS float atomics run about
+// four times slower on gfx950), in units of 2^-k with k from the largest possible term:
+// the sum is order-independent, hence reproducible, and each element differs from the
+// exact sum of its float terms by at most n*2^-(k+1) for n terms before the final rounding
+// to float.
+constexpr int COL_THREADS = 512;
+
+__device__ inline float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, const float *__restrict__ sings,
+                                                                    float tauftime, float *__restrict__ cols) {
+    extern __shared__ unsigned long long col[];
+    __shared__ int2 seg[64];    // {first plane | last plane << 16, file offset}
+    const int bin = blockIdx.x, tid = threadIdx.x;
+    for (int i = tid; i < t.nplanes; i += COL_THREADS) col[i] = 0ull;
+    const int kexp = fixed_point_exponent(sings[NCRYS], tauftime);
+    const float up = ldexpf(1.0f, kexp);
+    if (tid < 64) {
+        const int4 e = t.seg[tid];
+        seg[tid] = make_int2(e.y < 0 ? 0x7fff : (e.x | (e.y << 16)), e.z);   // 0x7fff: first > last, no plane
+    }
+    const int k0 = t.lor_off[bin], k1 = t.lor_off[bin + 1];
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int row = wave >> 1;
+    const int by = (wave & 1) * 64 + lane;
+    const bool live = by < NY;
+    const int w = t.window;
+    const float2 z0 = t.z[lane];                                          // A rings 0..63
+    const float2 z1 = lane < NY - 64 ? t.z[64 + lane] : make_float2(0.0f, 0.0f);   // A rings 64..103
+    const float zb = live ? t.z[by].x : 0.0f;
+    __syncthreads();
+    if (k0 == k1) goto store;
+    {
+        int4 L = t.lor[k0];
+        float sb = live ? sings[L.y + by] : 0.0f;
+        float sa0 = sings[L.x + lane];
+        float sa1 = lane < NY - 64 ? sings[L.x + 64 + lane] : 0.0f;
+        for (int k = k0; k < k1; k++) {
+            const float sd = __int_as_float(L.z), sz = __int_as_float(L.w);
+            const float b = sb, a0 = sa0, a1 = sa1;
+            if (k + 1 < k1) {     // next LOR's loads overlap this LOR's rings
+                L = t.lor[k + 1];
+                sb = live ? sings[L.y + by] : 0.0f;
+                sa0 = sings[L.x + lane];
+                sa1 = lane < NY - 64 ? sings[L.x + 64 + lane] : 0.0f;
+            }
+            // one crystal pair per lane for A ring ay: the predicate and target of its add
+            auto prepare = [&](int ay, float sa, float za, float cay, bool &ok, int &idx, float &val) {
+                const float awt = sa * tauftime;
+                const int dr = by - ay;
+                const float dz = zb - za;
+                const int plane = (int)(cay + sz * dz);
+                const float s = 0.5f + dz * sd;
+                const int si = (int)s;
+                const int code = s < 0.0f ? 1 - 2 * si : 2 * si;
+                // codes outside the 63-row map read row 63, which holds no plane; a valid
+                // row's planes lie inside the map's 207
+                const long long e = *reinterpret_cast<const long long *>(&seg[min((unsigned)code, (unsigned)SEGMAP)]);
+                const int first = (int)(e & 0xffff), last = (int)((e >> 16) & 0xffff);
+                ok = live & (awt != 0.0f) & (dr <= w) & (dr >= -w) & (plane >= first) & (plane <= last);
+                idx = plane + (int)(e >> 32);
+                val = b * awt;
+            };
+            // A rings row, row+4, ..., two per step: 16 below ring 64 and 10 from it (row < 4)
+#pragma unroll 2
+            for (int i = 0; i < 16; i += 2) {
+                const int ay = row + 4 * i;
+                const float sA = lane_value(a0, ay), sB = lane_value(a0, ay + 4);
+                if (sA * tauftime == 0.0f && sB * tauftime == 0.0f) continue;
+                bool okA, okB;
+                int iA, iB;
+                float vA, vB;
+                prepare(ay, sA, lane_value(z0.x, ay), lane_value(z0.y, ay), okA, iA, vA);
+                prepare(ay + 4, sB, lane_value(z0.x, ay + 4), lane_value(z0.y, ay + 4), okB, iB, vB);
+                if (okA) atomicAdd(&col[iA], to_fixed(vA, up));
+                if (okB) atomicAdd(&col[iB], to_fixed(vB, up));
+            }
+#pragma unroll 1
+            for (int i = 0; i < 10; i += 2) {
+                const int ay = 64 + row + 4 * i, r = row + 4 * i;
+                const float sA = lane_value(a1, r), sB = lane_value(a1, r + 4);
+                if (sA * tauftime == 0.0f && sB * tauftime == 0.0f) continue;
+                bool okA, okB;
+                int iA, iB;
+                float vA, vB;
+                prepare(ay, sA, lane_value(z1.x, r), lane_value(z1.y, r), okA, iA, vA);
+                prepare(ay + 4, sB, lane_value(z1.x, r + 4), lane_value(z1.y, r + 4), okB, iB, vB);
+                if (okA) atomicAdd(&col[iA], to_fixed(vA, up));
+                if (okB) atomicAdd(&col[iB], to_fixed(vB, up));
+            }
+        }
+    }
+store:
+    __syncthreads();
+    float *dst = cols + (size_t)bin * t.nplanes;
+    const double down = ldexp(1.0, -kexp);
+    for (int i = tid; i < t.nplanes; i += COL_THREADS) dst[i] = (float)((double)(long long)col[i] * down);
+}
+
+// [nsino][nplanes] -> [nplanes][nsino]; 64 x 64 tiles through LDS, 256 threads (4 rows each)
+__global__ __launch_bounds__(256) void column_transpose_kernel(const float *__restrict__ cols, float *__restrict__ out,
+                                                               int nsino, int nplanes) {
+    __shared__ float tile[64][65];
+    const int p0 = blockIdx.x * 64, b0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int b = b0 + r, p = p0 + tx;
+        tile[r][tx] = (b < nsino && p < nplanes) ? cols[(size_t)b * nplanes + p] : 0.0f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int p = p0 + r, b = b0 + tx;
+        if (p < nplanes && b < nsino) out[(size_t)p * nsino + b] = tile[tx][r];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host context
+
+struct hrrt_delays {
+    int span = 9, maxrd = 67, lr_type = 0;
+    int nprojs = 256, nviews = 288, nsino = 0, nplanes = 0, nsegs = 0;
+    float tau = 6e-9f, depth = 1.0f;
+    double binsize = 0, plane_sep = 0, crystal_radius = 0, d_tan_theta = 0;
+    double sin_head[NHEADS]{}, cos_head[NHEADS]{};
+    std::vector<Sol> lut;                    // [21][144][144], module pair 0 unused
+    float zpos[NY]{}, zpos2[NY]{};
+    int segz0[SEGMAP]{}, segzmax[SEGMAP]{}, segoff[SEGMAP]{};
+    long long pairs_full = 0;
+    int nlor = 0, device = 0;
+    std::vector<int4> h_lor;                 // LORs grouped by bin (device copy d_lor)
+    std::vector<int> h_lor_off;
+    int4 *d_lor = nullptr;
+    int *d_lor_off = nullptr;
+    int4 *d_seg = nullptr;
+    float2 *d_z = nullptr;
+    float *d_sings = nullptr;                // transposed singles [layer][head][x][ring], then max |singles|
+    float *d_cols = nullptr;                 // [nsino][nplanes] columns
+    int lds_bytes = 0;
+
+    // calc_det_to_phy geometry_info.cpp:137-163: block/crystal offsets in float, the head
+    // rotation in double, stored as float (constants geometry_info.h:51-55 are floats)
+    void crystal(int head, int layer, int detx, int dety, float pos[3]) const {
+        const float csize = 0.22f, cgap = 0.02f, bgap = 0.05f;
+        const float bsize = (float)(8.0 * (double)csize + (double)(7.0f * cgap));
+        const float xhsize = (float)(9.0 * (double)bsize + (double)(8.0f * bgap));
+        const float bx = (float)(detx / 8) * (bsize + bgap) + (float)(detx % 8) * (csize + cgap);
+        const double x = (double)bx - (double)xhsize / 2.0 + (double)csize / 2.0;
+        const double y = crystal_radius + (double)(depth * (float)(1 - layer)) + 0.5 * (double)depth;
+        const float bz = (float)(dety / 8) * (bsize + bgap) + (float)(dety % 8) * (csize + cgap);
+        const double s = sin_head[head], c = cos_head[head];
+        pos[0] = (float)(x * c + y * s);
+        pos[1] = (float)(-x * s + y * c);
+        pos[2] = bz;
+    }
+
+    // init_phy_to_pro lor_sinogram_map.cpp:99-138
+    Sol lor(const float a[3], const float b[3]) const {
+        const double dy = (double)(a[1] - b[1]), dx = (double)(a[0] - b[0]);
+        double d = std::sqrt(dx * dx + dy * dy);
+        float r = (float)((double)(a[1] * b[0] - a[0] * b[1]) / d);
+        float phi = (float)std::atan2(dx, dy);
+        if (phi < 0.0f) {
+            phi = phi + (float)M_PI;
+            r = -r;
+            d = -d;
+        }
+        if ((double)phi >= M_PI) {
+            phi = 0.0f;
+            r = -r;
+            d = -d;
+        }
+        int view = (int)((double)((float)nviews * phi) / M_PI);
+        if (view >= 288 || view < 0) view = 0;
+        const float z = (float)(((double)a[0] * dx + (double)a[1] * dy) / (d * d));
+        const int bin = (int)((double)(nprojs / 2) + ((double)r / binsize + 0.5));
+        Sol s;
+        s.nsino = (bin < 0 || bin > nprojs - 1) ? -1 : bin + view * nprojs;
+        s.d = (float)(1.0 / d / d_tan_theta);
+        s.z = (float)((double)z / plane_sep);
+        return s;
+    }
+
+    // segment sizes of one span (init_seginfo segment_info_fromarchive.cpp:35-78): first
+    // plane, last plane and file offset per segment; returns the plane total
+    int segments(int sp, int *z0, int *zmax, int *off) const {
+        const int n = 2 * (maxrd / sp) + 1, np = 2 * NY - 1, half = (sp + 1) / 2;
+        int total = 0;
+        for (int i = 0; i < n; i++) {
+            z0[i] = i == 0 ? 0 : half + sp * ((i - 1) / 2);
+            const int nz = np - 2 * z0[i];
+            zmax[i] = z0[i] + nz - 1;
+            off[i] = total - z0[i];
+            total += nz;
+        }
+        return total;
+    }
+};
+
+namespace {
+
+int make_tables(hrrt_delays &c, const hrrt_delays_params &p, std::string &err) {
+    if (p.span != 3 && p.span != 9) return err = "span must be 3 or 9", HRRT_ERR_ARG;
+    if (p.maxrd < 0 || p.maxrd > NY - 1) return err = "maxrd out of range", HRRT_ERR_ARG;
+    if (p.lr_type < 0 || p.lr_type > 2) return err = "lr_type must be 0, 1 or 2", HRRT_ERR_ARG;
+    c.span = p.span;
+    c.maxrd = p.maxrd;
+    c.lr_type = p.lr_type;
+    c.tau = p.tau > 0.0f ? p.tau : 6.0e-9f;
+    // init_geometry_hrrt geometry_info.cpp:169-221
+    const float pitch = p.pitch > 0.0f ? p.pitch : 0.24375f;
+    const float diam = p.diam > 0.0f ? p.diam : 46.9f;
+    c.depth = p.thick > 0.0f ? p.thick : 1.0f;
+    for (int i = 0; i < NHEADS; i++) {
+        c.sin_head[i] = std::sin(i * 2.0 * M_PI / NHEADS);
+        c.cos_head[i] = std::cos(i * 2.0 * M_PI / NHEADS);
+    }
+    c.crystal_radius = diam / 2.0;
+    if (c.lr_type == 0) {
+        c.nprojs = 256, c.nviews = 288, c.binsize = pitch / 2.0, c.plane_sep = pitch / 2.0;
+    } else if (c.lr_type == 1) {
+        c.nprojs = 160, c.nviews = 144, c.binsize = 0.2f, c.plane_sep = pitch;
+    } else {
+        c.nprojs = 128, c.nviews = 144, c.binsize = pitch, c.plane_sep = pitch;
+    }
+    c.nsino = c.nprojs * c.nviews;
+    // init_segment_info segment_info_fromarchive.cpp:118-141: planes of the span; the plane
+    // map is span 3, with span-9 file offsets through the conversion table
+    int z0[SEGMAP], zmax[SEGMAP], off[SEGMAP];
+    c.nsegs = 2 * (c.maxrd / 3) + 1;
+    if (c.nsegs > 45) return err = "maxrd above 68 leaves the segment tables", HRRT_ERR_ARG;
+    c.nplanes = c.segments(c.span, z0, zmax, off);
+    c.segments(3, c.segz0, c.segzmax, c.segoff);
+    if (c.span == 9) {
+        const int n9 = 2 * (c.maxrd / 9) + 1;
+        for (int i = 0; i < c.nsegs; i++) {
+            if (kS3toS9[i] >= n9) return err = "span-9 segment table too short for maxrd", HRRT_ERR_ARG;
+            c.segoff[i] = off[kS3toS9[i]];
+        }
+    }
+    c.d_tan_theta = 3 * c.plane_sep / c.crystal_radius;
+    return HRRT_OK;
+}
+
+// init_sol lor_sinogram_map.cpp:141-209: the LUT and the ring z tables from the geometry
+void compute_lut(hrrt_delays &c) {
+    c.lut.assign((size_t)(NMP + 1) * NXD * NXD, Sol{-1, 0.0f, 0.0f});
+    for (int mp = 1; mp <= NMP; mp++) {
+        for (int al = 0; al < NDOIS; al++)
+            for (int ax = 0; ax < NX; ax++) {
+                float pa[3];
+                c.crystal(kMpairs[mp][0], al, ax, 0, pa);
+                Sol *row = &c.lut[((size_t)mp * NXD + ax + NX * al) * NXD];
+                for (int bl = 0; bl < NDOIS; bl++)
+                    for (int bx = 0; bx < NX; bx++) {
+                        float pb[3];
+                        c.crystal(kMpairs[mp][1], bl, bx, 0, pb);
+                        row[bx + NX * bl] = c.lor(pa, pb);
+                    }
+            }
+    }
+    for (int i = 0; i < NY; i++) {
+        float pos[3];
+        c.crystal(0, 0, 0, i, pos);
+        c.zpos[i] = pos[2];
+        c.zpos2[i] = (float)((double)pos[2] / c.plane_sep + 0.5);
+    }
+}
+
+// init_lut_sol lor_sinogram_map.cpp:224-303
+int read_lut(hrrt_delays &c, const char *path, std::string &err) {
+    FILE *f = fopen(path, "rb");
+    if (!f) return err = std::string("cannot open LUT file ") + path, HRRT_ERR_ARG;
+    c.lut.assign((size_t)(NMP + 1) * NXD * NXD, Sol{-1, 0.0f, 0.0f});
+    const size_t n = (size_t)NMP * NXD * NXD;
+    bool ok = fread(&c.lut[(size_t)NXD * NXD], sizeof(Sol), n, f) == n;
+    ok = ok && fread(c.zpos, sizeof(float), NY, f) == (size_t)NY && fread(c.zpos2, sizeof(float), NY, f) == (size_t)NY;
+    fclose(f);
+    if (!ok) return err = std::string("short LUT file ") + path, HRRT_ERR_ARG;
+    return HRRT_OK;
+}
+
+int fail_d(int code, const std::string &msg) { return hrrt::set_last_error(code, msg.c_str()); }
+
+#define DTRY(expr)                                                                                        \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return fail_d(HRRT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Group the valid LORs by sinogram bin (host tables; uploaded by the first smoothing).
+int group_lors(hrrt_delays &c) {
+    std::vector<int> &count = c.h_lor_off;
+    count.assign(c.nsino + 1, 0);
+    for (int mp = 1; mp <= NMP; mp++)
+        for (int i = 0; i < NXD * NXD; i++) {
+            const Sol &s = c.lut[(size_t)mp * NXD * NXD + i];
+            if (s.nsino < -1 || s.nsino >= c.nsino)
+                return fail_d(HRRT_ERR_ARG, "LUT bin outside the sinogram (LUT built for another LR type?)");
+            if (s.nsino >= 0) count[s.nsino + 1]++;
+        }
+    for (int b = 0; b < c.nsino; b++) count[b + 1] += count[b];
+    c.nlor = count[c.nsino];
+    std::vector<int4> &lor = c.h_lor;
+    lor.assign(std::max(c.nlor, 1), make_int4(0, 0, 0, 0));
+    std::vector<int> fill(count.begin(), count.end() - 1);
+    // per A ring, the B rings inside |ay - by| < maxrd + 6 (gen_delays_fromarchive.cpp:175-184)
+    const int window = c.maxrd + 5;
+    long long ring_pairs = 0;
+    for (int ay = 0; ay < NY; ay++) ring_pairs += std::min(NY - 1, ay + window) - std::max(0, ay - window) + 1;
+    c.pairs_full = 0;
+    for (int mp = 1; mp <= NMP; mp++) {
+        const int ahead = kMpairs[mp][0], bhead = kMpairs[mp][1];
+        for (int axx = 0; axx < NXD; axx++)
+            for (int bxx = 0; bxx < NXD; bxx++) {
+                const Sol &s = c.lut[((size_t)mp * NXD + axx) * NXD + bxx];
+                if (s.nsino < 0) continue;
+                const int al = axx / NX, ax = axx % NX, bl = bxx / NX, bx = bxx % NX;
+                int4 e;
+                e.x = ((al * NHEADS + ahead) * NX + ax) * NY;
+                e.y = ((bl * NHEADS + bhead) * NX + bx) * NY;
+                memcpy(&e.z, &s.d, 4);
+                memcpy(&e.w, &s.z, 4);
+                lor[fill[s.nsino]++] = e;
+                c.pairs_full += ring_pairs;
+            }
+    }
+    c.lds_bytes = (int)sizeof(unsigned long long) * c.nplanes;
+    if (c.lds_bytes + 1024 > 160 * 1024) return fail_d(HRRT_ERR_UNSUPPORTED, "sinogram column exceeds LDS");
+    return HRRT_OK;
+}
+
+// Device copies of the tables, on the current device, at the first smoothing.
+int upload(hrrt_delays &c) {
+    if (c.d_lor) return HRRT_OK;
+    const std::vector<int4> &lor = c.h_lor;
+    const std::vector<int> &count = c.h_lor_off;
+    std::vector<int4> seg(64);
+    for (int i = 0; i < 64; i++) {
+        if (i < c.nsegs) seg[i] = make_int4(c.segz0[i], c.segzmax[i], c.segoff[i], 0);
+        else seg[i] = make_int4(1 << 30, -1, 0, 0);     // no plane: the map's -1 rows
+    }
+    std::vector<float2> z(NY);
+    for (int i = 0; i < NY; i++) z[i] = make_float2(c.zpos[i], c.zpos2[i]);
+    DTRY(hipGetDevice(&c.device));
+    DTRY(hipMalloc(&c.d_lor, sizeof(int4) * lor.size()));
+    DTRY(hipMalloc(&c.d_lor_off, sizeof(int) * count.size()));
+    DTRY(hipMalloc(&c.d_seg, sizeof(int4) * 64));
+    DTRY(hipMalloc(&c.d_z, sizeof(float2) * NY));
+    DTRY(hipMalloc(&c.d_sings, sizeof(float) * (NCRYS + 1)));
+    DTRY(hipMemcpy(c.d_lor, lor.data(), sizeof(int4) * lor.size(), hipMemcpyHostToDevice));
+    DTRY(hipMemcpy(c.d_lor_off, count.data(), sizeof(int) * count.size(), hipMemcpyHostToDevice));
+    DTRY(hipMemcpy(c.d_seg, seg.data(), sizeof(int4) * 64, hipMemcpyHostToDevice));
+    DTRY(hipMemcpy(c.d_z, z.data(), sizeof(float2) * NY, hipMemcpyHostToDevice));
+    if (c.lds_bytes > 64 * 1024)
+        DTRY(hipFuncSetAttribute((const void *)delays_column_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 c.lds_bytes));
+    return HRRT_OK;
+}
+
+// Head totals of the singles (compute_drate gen_delays_lib.cpp:168-176): layer, x, ring order.
+void head_sums(const float *srate, float *hsum) {
+    for (int head = 0; head < NHEADS; head++) {
+        float s = 0.0f;
+        for (int layer = 0; layer < NDOIS; layer++)
+            for (int cx = 0; cx < NX; cx++)
+                for (int cy = 0; cy < NY; cy++) s += srate[NX * head + cx + HX * cy + XYH * layer];
+        hsum[head] = s;
+    }
+}
+
+// Sum over the five heads in coincidence with `head` (gen_delays_lib.cpp:180-186).
+float fan_sum(const float *hsum, int head) {
+    float s = 0.0f;
+    for (int j = 0; j < 5; j++) s += hsum[(head + j + 2) % NHEADS];
+    return s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" int hrrt_delays_create(const hrrt_delays_params *params, hrrt_delays **out) {
+    if (!params || !out) return fail_d(HRRT_ERR_ARG, "null argument");
+    *out = nullptr;
+    hrrt_delays *c = new (std::nothrow) hrrt_delays();
+    if (!c) return fail_d(HRRT_ERR_NOMEM, "host allocation failed");
+    std::string err;
+    int r = make_tables(*c, *params, err);
+    if (r == HRRT_OK) {
+        if (params->lut_file) r = read_lut(*c, params->lut_file, err);
+        else compute_lut(*c);
+    }
+    if (r != HRRT_OK) {
+        delete c;
+        return fail_d(r, err);
+    }
+    if ((r = group_lors(*c)) != HRRT_OK) {
+        delete c;
+        return r;
+    }
+    *out = c;
+    return HRRT_OK;
+}
+
+extern "C" void hrrt_delays_destroy(hrrt_delays *c) {
+    if (!c) return;
+    for (void *p : {(void *)c->d_lor, (void *)c->d_lor_off, (void *)c->d_seg, (void *)c->d_z, (void *)c->d_sings,
+                    (void *)c->d_cols})
+        if (p) (void)hipFree(p);
+    delete c;
+}
+
+extern "C" int hrrt_delays_get_dims(const hrrt_delays *c, int *nprojs, int *nviews, int *nplanes, int *nsegs,
+                                    long long *pairs_full) {
+    if (!c) return fail_d(HRRT_ERR_ARG, "null context");
+    if (nprojs) *nprojs = c->nprojs;
+    if (nviews) *nviews = c->nviews;
+    if (nplanes) *nplanes = c->nplanes;
+    if (nsegs) *nsegs = c->nsegs;
+    if (pairs_full) *pairs_full = c->pairs_full;
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_delays_get_lut(const hrrt_delays *c, int mp, int32_t *nsino, float *d, float *z) {
+    if (!c) return fail_d(HRRT_ERR_ARG, "null context");
+    if (mp < 1 || mp > NMP) return fail_d(HRRT_ERR_ARG, "module pair must be 1..20");
+    const Sol *s = &c->lut[(size_t)mp * NXD * NXD];
+    for (int i = 0; i < NXD * NXD; i++) {
+        if (nsino) nsino[i] = s[i].nsino;
+        if (d) d[i] = s[i].d;
+        if (z) z[i] = s[i].z;
+    }
+    return HRRT_OK;
+}
+
+// save_lut_sol lor_sinogram_map.cpp:305-340
+extern "C" int hrrt_delays_save_lut(const hrrt_delays *c, const char *path) {
+    if (!c || !path) return fail_d(HRRT_ERR_ARG, "null argument");
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail_d(HRRT_ERR_ARG, std::string("cannot create ") + path);
+    const size_t n = (size_t)NMP * NXD * NXD;
+    bool ok = fwrite(&c->lut[(size_t)NXD * NXD], sizeof(Sol), n, f) == n;
+    ok = ok && fwrite(c->zpos, sizeof(float), NY, f) == (size_t)NY && fwrite(c->zpos2, sizeof(float), NY, f) == (size_t)NY;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? HRRT_OK : fail_d(HRRT_ERR_ARG, std::string("write failed: ") + path);
+}
+
+// compute_csings_from_drates gen_delays_lib.cpp:254-271 with compute_initial_srate (:233-241),
+// compute_drate (:162-196), errtotal (:244-252) and estimate_srate (:198-230).  The error
+// sum is 64-bit (the reference's int overflows on large errors; only its zero test matters).
+extern "C" int hrrt_delays_singles(const hrrt_delays *c, const int32_t *coins, float scan_duration, float *singles,
+                                   int *niter) {
+    if (!c || !coins || !singles) return fail_d(HRRT_ERR_ARG, "null argument");
+    const float dt = scan_duration > 0.0f ? scan_duration : 1.0f, tau = c->tau;
+    const int32_t *dcoins = coins + NCRYS;
+    std::vector<float> drates(NCRYS), xrates(NCRYS);
+    float dtotal = 0.0f;
+    for (int i = 0; i < NCRYS; i++) drates[i] = (float)dcoins[i] / dt;
+    for (int i = 0; i < NCRYS; i++) dtotal += drates[i];
+    const float stotal = (float)(8.0 * std::sqrt((double)dtotal / (40. * (double)tau)));
+    for (int i = 0; i < NCRYS; i++) singles[i] = drates[i] * stotal / dtotal;
+    float hsum[NHEADS];
+    int iter;
+    for (iter = 0; iter < 100; iter++) {
+        head_sums(singles, hsum);
+        long long err = 0;
+        for (int head = 0; head < NHEADS; head++) {
+            const float fan = fan_sum(hsum, head);
+            for (int layer = 0; layer < NDOIS; layer++)
+                for (int cx = 0; cx < NX; cx++)
+                    for (int cy = 0; cy < NY; cy++) {
+                        const int i = NX * head + cx + HX * cy + XYH * layer;
+                        xrates[i] = singles[i] * tau * fan;
+                    }
+        }
+        for (int i = 0; i < NCRYS; i++) {
+            const long long e = (int)((double)dcoins[i] - (0.5 + (double)(xrates[i] * dt)));
+            err += e * e;
+        }
+        if (iter && err == 0) break;
+        head_sums(singles, hsum);
+        for (int head = 0; head < NHEADS; head++) {
+            const float fan = fan_sum(hsum, head);
+            for (int layer = 0; layer < NDOIS; layer++)
+                for (int cx = 0; cx < NX; cx++)
+                    for (int cy = 0; cy < NY; cy++) {
+                        const int i = NX * head + cx + HX * cy + XYH * layer;
+                        singles[i] = (singles[i] + drates[i] / (tau * fan)) / 2.0f;
+                    }
+        }
+    }
+    if (niter) *niter = iter;
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_delays_smooth(hrrt_delays *c, const float *d_singles, float scan_duration, float *d_delays,
+                                  void *stream) {
+    if (!c || !d_singles || !d_delays) return fail_d(HRRT_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    int r = upload(*c), dev = -1;
+    if (r != HRRT_OK) return r;
+    DTRY(hipGetDevice(&dev));
+    if (dev != c->device) return fail_d(HRRT_ERR_ARG, "context tables live on another device");
+    if (!c->d_cols) DTRY(hipMalloc(&c->d_cols, sizeof(float) * (size_t)c->nsino * c->nplanes));
+    const float ftime = scan_duration > 0.0f ? scan_duration : 1.0f;
+    const float tauftime = c->tau * ftime;
+    Tables t{c->d_lor, c->d_lor_off, c->d_seg, c->d_z, c->nsino, c->nplanes, c->maxrd + 5};
+    sings_prepare_kernel<<<1, PREP_THREADS, 0, s>>>(d_singles, c->d_sings);
+    DTRY(hipGetLastError());
+    delays_column_kernel<<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, c->d_cols);
+    DTRY(hipGetLastError());
+    column_transpose_kernel<<<dim3((c->nplanes + 63) / 64, (c->nsino + 63) / 64), 256, 0, s>>>(c->d_cols, d_delays,
+                                                                                               c->nsino, c->nplanes);
+    DTRY(hipGetLastError());
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_gen_delays(hrrt_delays *c, const int32_t *coins, float scan_duration, float *d_delays,
+                               float *singles_out, int *niter, void *stream) {
+    if (!c || !coins || !d_delays) return fail_d(HRRT_ERR_ARG, "null argument");
+    std::vector<float> sg(NCRYS);
+    int r = hrrt_delays_singles(c, coins, scan_duration, sg.data(), niter);
+    if (r != HRRT_OK) return r;
+    if (singles_out) memcpy(singles_out, sg.data(), sizeof(float) * NCRYS);
+    // the singles are staged in the head of the output (nplanes*nsino >= crystals), which the
+    // smoothing reads into its transposed copy before its last kernel overwrites the output
+    hipStream_t s = (hipStream_t)stream;
+    DTRY(hipMemcpyAsync(d_delays, sg.data(), sizeof(float) * NCRYS, hipMemcpyHostToDevice, s));
+    DTRY(hipStreamSynchronize(s));
+    return hrrt_delays_smooth(c, d_delays, scan_duration, d_delays, stream);
+}
+
+extern "C" int hrrt_delays_write(hrrt_delays *c, const float *d_delays, const char *path, void *stream) {
+    if (!c || !d_delays || !path) return fail_d(HRRT_ERR_ARG, "null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t plane = (size_t)c->nsino;
+    std::vector<float> h(plane * 64);
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail_d(HRRT_ERR_ARG, std::string("cannot create ") + path);
+    bool ok = true;
+    for (int p0 = 0; ok && p0 < c->nplanes; p0 += 64) {
+        const size_t n = plane * std::min(64, c->nplanes - p0);
+        if (hipMemcpyAsync(h.data(), d_delays + (size_t)p0 * plane, sizeof(float) * n, hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            fclose(f);
+            return fail_d(HRRT_ERR_HIP, "copy of the delays to the host failed");
+        }
+        ok = fwrite(h.data(), sizeof(float), n, f) == n;
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? HRRT_OK : fail_d(HRRT_ERR_ARG, std::string("write failed: ") + path);
+}

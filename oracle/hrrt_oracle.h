@@ -177,6 +177,44 @@ void     og_calculate_normfac(const og_geom *g, float *invnorm, float *normfac_a
 void     og_osem_subset(const og_geom *g, float *image, float *correction, const float *normfac_subset,
                         const float *add, const int16_t *prompt, int isubset, int nthreads, int workers);
 
+/* ---- smoothed randoms (gen_delays), randoms.c ---- */
+#define OD_NHEADS   8
+#define OD_NDOIS    2
+#define OD_NX       72
+#define OD_NY       104
+#define OD_NXD      (OD_NX * OD_NDOIS)                      /* crystals of a head row, both layers */
+#define OD_NCRYS    (OD_NHEADS * OD_NDOIS * OD_NX * OD_NY)  /* 119808, [layer][y][head][x] */
+#define OD_NMP      20                                      /* module pairs */
+#define OD_NSEGMAP  63
+#define OD_NPLANEMAP (2 * OD_NY - 1)
+
+typedef struct { int nsino; float d; float z; } og_sol;     /* SOL lor_sinogram_map.h:18 */
+
+typedef struct og_delays {
+    int span, maxrd, lr_type;
+    int nprojs, nviews, nsino, nplanes, nsegs;
+    double binsize, plane_sep, crystal_radius, d_tan_theta;
+    double sin_head[OD_NHEADS], cos_head[OD_NHEADS];
+    float depth;
+    float czpos[OD_NY], czpos2[OD_NY];
+    int segz0[OD_NSEGMAP], segzmax[OD_NSEGMAP], segzoffset[OD_NSEGMAP];
+    short segplane[OD_NSEGMAP][OD_NPLANEMAP];
+    og_sol *sol;                                            /* [21][144][144]; module pair 0 unused */
+} og_delays;
+
+og_delays *og_delays_create(int span, int maxrd, int lr_type, float pitch, float diam, float thick);
+void      og_delays_destroy(og_delays *g);
+int       og_delays_mpair(int mp, int *ahead, int *bhead);
+void      og_delays_dims(const og_delays *g, int *out /* nprojs nviews nplanes nsegs nsino span */);
+void      og_delays_crystal(const og_delays *g, int head, int layer, int detx, int dety, float pos[3]);
+void      og_delays_phy_to_pro(const og_delays *g, const float a[3], const float b[3], og_sol *sol);
+void      og_delays_sol(const og_delays *g, int mp, int *nsino, float *d, float *z);   /* [144][144] */
+void      og_delays_tables(const og_delays *g, float *zpos, float *zpos2, short *segplane);
+void      og_delays_drate(const float *srate, float tau, float *drate);
+int       og_delays_singles(const int *dcoins, float tau, float dt, float *srates);
+long long og_delays_smooth(const og_delays *g, const float *csings, float tau, float ftime, const int *mps, int nmp,
+                           float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -402,3 +402,115 @@ class Oracle:
     def osem_subset_psf(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
         self.L.og_osem_subset_psf(self.gp, self.psf, _ptr(image), _ptr(correction), _ptr(normfac_subset),
                                   _ptr(add), _ptr(prompt), isubset, nthreads, workers)
+
+
+NCRYS = 8 * 2 * 72 * 104          # crystals, [layer][y][head][x]
+NXD = 144                         # crystals of a head row over both layers
+
+
+class Sol(C.Structure):
+    _fields_ = [("nsino", C.c_int), ("d", C.c_float), ("z", C.c_float)]
+
+
+def _delays_lib():
+    L = lib()
+    if not getattr(L, "_delays_bound", False):
+        vp = C.c_void_p
+        L.og_delays_create.restype = vp
+        L.og_delays_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float]
+        L.og_delays_destroy.argtypes = [vp]
+        L.og_delays_mpair.restype = C.c_int
+        L.og_delays_mpair.argtypes = [C.c_int, _PI, _PI]
+        L.og_delays_dims.argtypes = [vp, vp]
+        L.og_delays_crystal.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
+        L.og_delays_phy_to_pro.argtypes = [vp, vp, vp, C.POINTER(Sol)]
+        L.og_delays_sol.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.og_delays_tables.argtypes = [vp, vp, vp, vp]
+        L.og_delays_drate.argtypes = [vp, C.c_float, vp]
+        L.og_delays_singles.restype = C.c_int
+        L.og_delays_singles.argtypes = [vp, C.c_float, C.c_float, vp]
+        L.og_delays_smooth.restype = C.c_longlong
+        L.og_delays_smooth.argtypes = [vp, vp, C.c_float, C.c_float, vp, C.c_int, vp]
+        L._delays_bound = True
+    return L
+
+
+class Delays:
+    """Smoothed randoms (gen_delays) restated on the CPU: geometry, LUT, singles, smoothing."""
+
+    def __init__(self, span=9, maxrd=67, lr_type=0, pitch=0.0, diam=0.0, thick=0.0):
+        self.L = _delays_lib()
+        self.g = self.L.og_delays_create(span, maxrd, lr_type, pitch, diam, thick)
+        if not self.g:
+            raise ValueError("invalid gen_delays parameters")
+        d = np.zeros(6, np.int32)
+        self.L.og_delays_dims(self.g, _ptr(d))
+        self.nprojs, self.nviews, self.nplanes, self.nsegs, self.nsino, self.span = (int(v) for v in d)
+
+    def close(self):
+        if self.g:
+            self.L.og_delays_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def mpair(mp):
+        a, b = C.c_int(), C.c_int()
+        if _delays_lib().og_delays_mpair(mp, C.byref(a), C.byref(b)) != 0:
+            raise ValueError(mp)
+        return a.value, b.value
+
+    def crystal(self, head, layer, x, y):
+        pos = np.zeros(3, np.float32)
+        self.L.og_delays_crystal(self.g, head, layer, x, y, _ptr(pos))
+        return pos
+
+    def phy_to_pro(self, a, b):
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        s = Sol()
+        self.L.og_delays_phy_to_pro(self.g, _ptr(a), _ptr(b), C.byref(s))
+        return s.nsino, np.float32(s.d), np.float32(s.z)
+
+    def sol(self, mp):
+        n = np.zeros((NXD, NXD), np.int32)
+        d = np.zeros((NXD, NXD), np.float32)
+        z = np.zeros((NXD, NXD), np.float32)
+        self.L.og_delays_sol(self.g, mp, _ptr(n), _ptr(d), _ptr(z))
+        return n, d, z
+
+    def tables(self):
+        zpos = np.zeros(104, np.float32)
+        zpos2 = np.zeros(104, np.float32)
+        segplane = np.zeros((63, 207), np.int16)
+        self.L.og_delays_tables(self.g, _ptr(zpos), _ptr(zpos2), _ptr(segplane))
+        return zpos, zpos2, segplane
+
+    @staticmethod
+    def drate(srate, tau):
+        srate = np.ascontiguousarray(srate, np.float32)
+        out = np.zeros(NCRYS, np.float32)
+        _delays_lib().og_delays_drate(_ptr(srate), tau, _ptr(out))
+        return out
+
+    @staticmethod
+    def singles(dcoins, tau, dt):
+        dcoins = np.ascontiguousarray(dcoins, np.int32)
+        assert dcoins.size == NCRYS
+        out = np.zeros(NCRYS, np.float32)
+        it = _delays_lib().og_delays_singles(_ptr(dcoins), tau, dt, _ptr(out))
+        return out, it
+
+    def smooth(self, csings, tau=6e-9, ftime=1.0, mps=None):
+        """Returns (delays [nplanes][nviews][nprojs], crystal pairs visited)."""
+        csings = np.ascontiguousarray(csings, np.float32)
+        assert csings.size == NCRYS
+        out = np.zeros((self.nplanes, self.nviews, self.nprojs), np.float32)
+        m = None if mps is None else np.ascontiguousarray(mps, np.int32)
+        pairs = self.L.og_delays_smooth(self.g, _ptr(csings), tau, ftime, _ptr(m), 0 if m is None else m.size, _ptr(out))
+        return out, int(pairs)

@@ -11,11 +11,11 @@ import pytest
 
 from conftest import GEOMS, ROOT
 
-HEADER = os.path.join(ROOT, "include", "hrrt_osem.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("hrrt_osem.h", "hrrt_randoms.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
+    text = "".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(hrrt_\w+)\s*\(", text)))
 
@@ -135,11 +135,12 @@ def test_null_arguments(lib):
     assert L.hrrt_get_dims(None, None) == lib.HRRT_ERR_ARG
     assert L.hrrt_query_geometry(None, None, *([None] * 6)) == lib.HRRT_ERR_ARG
     for name, (_, args) in lib.SIGNATURES.items():
-        if name in ("hrrt_destroy", "hrrt_last_error", "hrrt_create", "hrrt_query_geometry"):
+        if name in ("hrrt_destroy", "hrrt_delays_destroy", "hrrt_last_error", "hrrt_create", "hrrt_query_geometry"):
             continue
         zero = [0 if a in (C.c_int, C.c_longlong) else (0.0 if a is C.c_float else None) for a in args]
         assert getattr(L, name)(*zero) == lib.HRRT_ERR_ARG, name
     L.hrrt_destroy(None)          # a no-op, like free(NULL)
+    L.hrrt_delays_destroy(None)
 
 
 def test_bench_refuses_more_than_one_gpu():
