@@ -158,6 +158,54 @@ def psf_leg(torch, g, image0, normfac_psf, prompt, add, order, steps):
             "note": "two convolve3d per subset (image, correction), sensitivity blurred before inversion"}
 
 
+def randoms_leg(torch, steps, seed, cpu_mps):
+    """Smoothed randoms (gen_delays) for the same span-3 sinogram: singles from a synthetic
+    coincidence histogram (delayed counts of the model d = tau s F T) on the host, then the
+    smoothing on the GPU.  CPU baseline: the oracle on the listed module pairs."""
+    import numpy as np
+    from hrrt_reconstruction_amd.randoms import CRYSTALS, GenDelays
+    rng = np.random.default_rng(seed)
+    tau, T = 6e-9, 600.0
+    s = rng.uniform(2e3, 6e3, CRYSTALS)                       # cps; dead-time-free model
+    head = s.reshape(2, 104, 8, 72).sum(axis=(0, 1, 3))
+    fan = np.array([sum(head[(h + j + 2) % 8] for j in range(5)) for h in range(8)])
+    dcoins = rng.poisson(tau * s * np.broadcast_to(fan[None, None, :, None], (2, 104, 8, 72)).reshape(-1) * T)
+    coins = np.concatenate([dcoins * 10, dcoins]).astype(np.int32)   # prompts (unused), delayed
+    g = GenDelays(span=GEOM["span"], maxrd=GEOM["maxdel"])
+    t0 = time.perf_counter()
+    singles, niter = g.singles(coins, T)
+    singles_ms = 1e3 * (time.perf_counter() - t0)
+    d_s = torch.from_numpy(singles).cuda()
+    out = g.empty()
+    for _ in range(2):
+        g.smooth(d_s, T, out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        g.smooth(d_s, T, out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    total = float(out.double().sum().item())
+    log(f"randoms: singles {singles_ms:.1f} ms ({niter} iterations), smoothing {ms:.3f} ms")
+    res = {"ms_per_smoothing": round(ms, 4), "crystal_pairs_per_s": round(g.pairs_full / (ms * 1e-3), 1),
+           "crystal_pairs": g.pairs_full, "singles_ms_host": round(singles_ms, 2), "singles_iterations": niter,
+           "span": g.span, "planes": g.nplanes, "sum": total, "cpu_baseline": None}
+    del out
+    torch.cuda.empty_cache()
+    if cpu_mps:
+        from oracle import oracle as O
+        D = O.Delays(span=GEOM["span"], maxrd=GEOM["maxdel"])
+        t0 = time.perf_counter()
+        _, pairs = D.smooth(singles, tau, T, mps=cpu_mps)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(pairs / dt, 1), "unit": "crystal pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"module pairs {cpu_mps} ({pairs} crystal pairs, {dt:.1f} s)"}
+        log(f"randoms CPU oracle: {pairs / dt / 1e6:.1f} M crystal pairs/s")
+    return res
+
+
 def batched_frames(torch, nframes, iterations, seed):
     """Dynamic study in batched mode: every frame's subsets run in one launch per kernel, each subset
     of an iteration replayed from a HIP graph.  FP32 form and FP16-input precision form."""
@@ -269,6 +317,8 @@ def main():
     ap.add_argument("--frame-iterations", type=int, default=2)
     ap.add_argument("--psf-steps", type=int, default=16,
                     help="timed subset updates with the PSF resolution model (-B, HRRT defaults; 0 = skip)")
+    ap.add_argument("--randoms-steps", type=int, default=10,
+                    help="timed smoothed-randoms (gen_delays) computations for the same span (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -355,6 +405,9 @@ def main():
     psf = None
     if args.psf_steps > 0:
         psf = psf_leg(torch, g, image0, normfac_psf, prompt, add, order, args.psf_steps)
+    randoms = None
+    if args.randoms_steps > 0:
+        randoms = randoms_leg(torch, args.randoms_steps, args.seed + 11, [1, 3] if args.cpu_subsets > 0 else None)
     batched = None
     if args.frames > 0:
         batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
@@ -385,6 +438,7 @@ def main():
                         "alg_bytes": alg[k]} for k, v in kt.items()},
         "cpu_baseline": cpu,
         "psf": psf,
+        "randoms": randoms,
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

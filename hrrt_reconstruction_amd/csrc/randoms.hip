@@ -6,11 +6,12 @@
 // z), the LUT grouped by sinogram bin, and the singles iteration.
 // Device side (per smoothing):
 //   sings_prepare_kernel    singles [layer][ring][head][x] -> [layer][head][x][ring], so that a
-//                           crystal's 104 rings are contiguous, and their largest magnitude
+//                           crystal's 104 rings are contiguous, and each column's largest magnitude
 //   delays_column_kernel    one workgroup per sinogram bin: every crystal pair of every LOR that
 //                           lands in the bin, accumulated into the bin's column of planes in
-//                           LDS (64-bit fixed point, ds_add_u64), then stored contiguously
-//   column_transpose_kernel [bin][plane] columns -> [plane][view][bin] output, 64 x 64 tiles
+//                           LDS (64-bit fixed point, ds_add_u64), then stored into the
+//                           plane-major output (span 9) or as a contiguous column (span 3)
+//   column_transpose_kernel span 3: [bin][plane] columns -> [plane][view][bin], 64 x 64 tiles
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,7 +51,7 @@ static_assert(sizeof(Sol) == 12, "LUT record is 12 bytes");
 struct Tables {    // device tables of one context
     const int4 *lor;      // [nlor] {A ring-column offset, B ring-column offset, bits(d), bits(z)}
     const int *lor_off;   // [nsino + 1] LORs of bin b: lor[lor_off[b] .. lor_off[b+1])
-    const int4 *seg;      // [64] {first plane, last plane, file offset, 0} per segment code
+    const int2 *seg;      // [128] {first | last << 16, file offset} by 2*(int)seg + (seg < 0) + 64
     const float2 *z;      // [104] {ring z (cm), ring z in planes + 0.5}
     int nsino, nplanes, window;
 };
@@ -58,42 +59,40 @@ struct Tables {    // device tables of one context
 // ---------------------------------------------------------------- kernels
 
 // One workgroup: singles [layer][ring][head][x] -> [layer][head][x][ring] (a crystal's 104 rings
-// contiguous) and the largest |singles| into out[NCRYS], which sets the fixed-point scale.
+// contiguous), then the largest |singles| of each such crystal column into out[NCRYS + column],
+// which sets the fixed-point scale of every sinogram bin.
 constexpr int PREP_THREADS = 1024;
+constexpr int NCOLS = NCRYS / NY;   // 1152 crystal columns
 
 __global__ __launch_bounds__(PREP_THREADS) void sings_prepare_kernel(const float *__restrict__ in,
                                                                      float *__restrict__ out) {
-    __shared__ float wmax[PREP_THREADS / 64];
-    float m = 0.0f;
     for (int i = threadIdx.x; i < NCRYS; i += PREP_THREADS) {
         const int ring = i % NY, x = (i / NY) % NX, head = (i / (NY * NX)) % NHEADS, layer = i / (NY * NX * NHEADS);
-        const float v = in[layer * XYH + ring * HX + head * NX + x];
-        out[i] = v;
-        m = fmaxf(m, fabsf(v));
+        out[i] = in[layer * XYH + ring * HX + head * NX + x];
     }
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < PREP_THREADS / 64; w++) m = fmaxf(m, wmax[w]);
-        out[NCRYS] = m;
+    for (int j = threadIdx.x; j < NCOLS; j += PREP_THREADS) {
+        const int x = j % NX, head = (j / NX) % NHEADS, layer = j / (NX * NHEADS);
+        const float *src = in + layer * XYH + head * NX + x;
+        float m = 0.0f;
+        for (int ring = 0; ring < NY; ring++) m = fmaxf(m, fabsf(src[ring * HX]));
+        out[NCRYS + j] = m;
     }
 }
 
-// Exponent k of the fixed-point unit 2^-k: every term s_A*tau*T*s_B is at most
-// M = smax^2*tau*T < 2^E, so a term scaled by 2^k = 2^(36-E) is below 2^36 and a sum of up
-// to 2^27 terms stays inside int64.
-__device__ inline int fixed_point_exponent(float smax, float tauftime) {
-    const float m = smax * smax * tauftime;
+// Exponent k of a bin's fixed-point unit 2^-k.  A term s_A*tau*T*s_B of the bin is at most
+// about m = max over its LORs of colmax(A)*colmax(B)*tau*T < 2^E, and an element of the bin
+// sums at most nlor*1664 terms (104 A rings, each with at most 16 B rings on one segment and
+// plane), so terms below 2^(62-bits) with 2^bits >= nlor*1664 keep every sum inside int64.
+__device__ inline int fixed_point_exponent(float m, int nlor) {
     int e = 0;
     if (m > 0.0f && m <= 3.0e38f) frexpf(m, &e);
-    return min(36 - e, 127);
+    const int bits = 32 - __clz(nlor * 1664);
+    return max(min(62 - bits - e, 127), -126);
 }
 
-// Term -> int64 in units of 2^-k, rounded to nearest: x = v*2^k is exact (a power of two),
-// and so are its split into x = hi*2^32 + lo (hi integral) and lo's rounding below 2^32.
-__device__ inline unsigned long long to_fixed(float v, float up) {
-    const float x = v * up;
+// Term -> int64 in units of 2^-k, rounded to nearest: x = v*2^k is exact (a power of two), and
+// so are its split into x = hi*2^32 + lo (hi integral) and lo's rounding below 2^32.
+__device__ inline unsigned long long to_fixed(float x) {
     const float hi = floorf(x * 0x1p-32f);
     const float lo = x - hi * 0x1p32f;
     return ((unsigned long long)(long long)(int)hi << 32) + (unsigned long long)(unsigned)rintf(lo);
@@ -113,24 +112,35 @@ __device__ inline unsigned long long to_fixed(float v, float up) {
 // exact sum of its float terms by at most n*2^-(k+1) for n terms before the final rounding
 // to float.
 constexpr int COL_THREADS = 512;
+// Columns up to this many planes (span 9: 2209) are stored straight into the plane-major
+// output; longer ones (span 3: 6367) go through bin-major columns and a tiled transpose,
+// since 96 workgroups' worth of them per XCD no longer gather in its L2.
+constexpr int DIRECT_PLANES = 4096;
 
 __device__ inline float lane_value(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+template <bool DIRECT>
 __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, const float *__restrict__ sings,
                                                                     float tauftime, float *__restrict__ cols) {
     extern __shared__ unsigned long long col[];
-    __shared__ int2 seg[64];    // {first plane | last plane << 16, file offset}
-    const int bin = blockIdx.x, tid = threadIdx.x;
+    // segment entry by e = 2*(int)seg + (seg < 0) + 64: {first plane | last plane << 16, file offset}
+    __shared__ int2 seg[128];
+    // XCD-aware: workgroups are dealt to the 8 XCDs in turn, so XCD x gets bins
+    // [x*nsino/8, (x+1)*nsino/8) in order and its L2 gathers whole lines of the plane-major output
+    const int per = gridDim.x >> 3;
+    const int bin = (blockIdx.x & 7) * per + (blockIdx.x >> 3), tid = threadIdx.x;
     for (int i = tid; i < t.nplanes; i += COL_THREADS) col[i] = 0ull;
-    const int kexp = fixed_point_exponent(sings[NCRYS], tauftime);
-    const float up = ldexpf(1.0f, kexp);
-    if (tid < 64) {
-        const int4 e = t.seg[tid];
-        seg[tid] = make_int2(e.y < 0 ? 0x7fff : (e.x | (e.y << 16)), e.z);   // 0x7fff: first > last, no plane
-    }
+    if (tid < 128) seg[tid] = t.seg[tid];
     const int k0 = t.lor_off[bin], k1 = t.lor_off[bin + 1];
+    float mlor = 0.0f;
+    for (int k = k0; k < k1; k++) {
+        const int4 L = t.lor[k];
+        mlor = fmaxf(mlor, sings[NCRYS + L.x / NY] * sings[NCRYS + L.y / NY]);
+    }
+    const int kexp = fixed_point_exponent(mlor * tauftime, max(k1 - k0, 1));
+    const float up = ldexpf(1.0f, kexp);
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int row = wave >> 1;
@@ -149,64 +159,67 @@ __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, co
         float sa1 = lane < NY - 64 ? sings[L.x + 64 + lane] : 0.0f;
         for (int k = k0; k < k1; k++) {
             const float sd = __int_as_float(L.z), sz = __int_as_float(L.w);
-            const float b = sb, a0 = sa0, a1 = sa1;
+            // the term s_B*(s_A*tau*T) scaled by 2^k: scaling s_B first is exact and rounds the
+            // same product (a power of two commutes with rounding)
+            const float bu = sb * up, a0 = sa0 * tauftime, a1 = sa1 * tauftime;
             if (k + 1 < k1) {     // next LOR's loads overlap this LOR's rings
                 L = t.lor[k + 1];
                 sb = live ? sings[L.y + by] : 0.0f;
                 sa0 = sings[L.x + lane];
                 sa1 = lane < NY - 64 ? sings[L.x + 64 + lane] : 0.0f;
             }
-            // one crystal pair per lane for A ring ay: the predicate and target of its add
-            auto prepare = [&](int ay, float sa, float za, float cay, bool &ok, int &idx, float &val) {
-                const float awt = sa * tauftime;
+            // one crystal pair per lane for A ring ay (A singles*tau*T awt, ring z za, plane
+            // cay): the predicate, column index and fixed-point value of its add.  The
+            // reference skips silent A crystals; their terms are exact zeros here.
+            auto prepare = [&](int ay, float awt, float za, float cay, bool &ok, int &idx, unsigned long long &q) {
                 const int dr = by - ay;
                 const float dz = zb - za;
                 const int plane = (int)(cay + sz * dz);
                 const float s = 0.5f + dz * sd;
                 const int si = (int)s;
-                const int code = s < 0.0f ? 1 - 2 * si : 2 * si;
-                // codes outside the 63-row map read row 63, which holds no plane; a valid
-                // row's planes lie inside the map's 207
-                const long long e = *reinterpret_cast<const long long *>(&seg[min((unsigned)code, (unsigned)SEGMAP)]);
-                const int first = (int)(e & 0xffff), last = (int)((e >> 16) & 0xffff);
-                ok = live & (awt != 0.0f) & (dr <= w) & (dr >= -w) & (plane >= first) & (plane <= last);
-                idx = plane + (int)(e >> 32);
-                val = b * awt;
+                // e - 64 = 2*(int)seg + (seg < 0) names the code (2*si for seg >= 0, 1 - 2*si
+                // for seg < 0); e outside [0, 128) and impossible pairs map to no plane
+                const int e = min(max(2 * si + (s < 0.0f ? 1 : 0), -64), 63);
+                const long long ent = *reinterpret_cast<const long long *>(&seg[64 + e]);
+                const int first = (int)(ent & 0xffff), last = (int)((ent >> 16) & 0xffff);
+                ok = live & (dr <= w) & (dr >= -w) & (plane >= first) & (plane <= last);
+                idx = plane + (int)(ent >> 32);
+                q = to_fixed(bu * awt);
             };
             // A rings row, row+4, ..., two per step: 16 below ring 64 and 10 from it (row < 4)
 #pragma unroll 2
             for (int i = 0; i < 16; i += 2) {
                 const int ay = row + 4 * i;
-                const float sA = lane_value(a0, ay), sB = lane_value(a0, ay + 4);
-                if (sA * tauftime == 0.0f && sB * tauftime == 0.0f) continue;
                 bool okA, okB;
                 int iA, iB;
-                float vA, vB;
-                prepare(ay, sA, lane_value(z0.x, ay), lane_value(z0.y, ay), okA, iA, vA);
-                prepare(ay + 4, sB, lane_value(z0.x, ay + 4), lane_value(z0.y, ay + 4), okB, iB, vB);
-                if (okA) atomicAdd(&col[iA], to_fixed(vA, up));
-                if (okB) atomicAdd(&col[iB], to_fixed(vB, up));
+                unsigned long long qA, qB;
+                prepare(ay, lane_value(a0, ay), lane_value(z0.x, ay), lane_value(z0.y, ay), okA, iA, qA);
+                prepare(ay + 4, lane_value(a0, ay + 4), lane_value(z0.x, ay + 4), lane_value(z0.y, ay + 4), okB, iB, qB);
+                if (okA) atomicAdd(&col[iA], qA);
+                if (okB) atomicAdd(&col[iB], qB);
             }
 #pragma unroll 1
             for (int i = 0; i < 10; i += 2) {
                 const int ay = 64 + row + 4 * i, r = row + 4 * i;
-                const float sA = lane_value(a1, r), sB = lane_value(a1, r + 4);
-                if (sA * tauftime == 0.0f && sB * tauftime == 0.0f) continue;
                 bool okA, okB;
                 int iA, iB;
-                float vA, vB;
-                prepare(ay, sA, lane_value(z1.x, r), lane_value(z1.y, r), okA, iA, vA);
-                prepare(ay + 4, sB, lane_value(z1.x, r + 4), lane_value(z1.y, r + 4), okB, iB, vB);
-                if (okA) atomicAdd(&col[iA], to_fixed(vA, up));
-                if (okB) atomicAdd(&col[iB], to_fixed(vB, up));
+                unsigned long long qA, qB;
+                prepare(ay, lane_value(a1, r), lane_value(z1.x, r), lane_value(z1.y, r), okA, iA, qA);
+                prepare(ay + 4, lane_value(a1, r + 4), lane_value(z1.x, r + 4), lane_value(z1.y, r + 4), okB, iB, qB);
+                if (okA) atomicAdd(&col[iA], qA);
+                if (okB) atomicAdd(&col[iB], qB);
             }
         }
     }
 store:
     __syncthreads();
-    float *dst = cols + (size_t)bin * t.nplanes;
     const double down = ldexp(1.0, -kexp);
-    for (int i = tid; i < t.nplanes; i += COL_THREADS) dst[i] = (float)((double)(long long)col[i] * down);
+    if (DIRECT) {   // plane-major output [nplanes][nsino]
+        for (int i = tid; i < t.nplanes; i += COL_THREADS) cols[(size_t)i * t.nsino + bin] = (float)((double)(long long)col[i] * down);
+    } else {        // bin-major columns [nsino][nplanes]
+        float *dst = cols + (size_t)bin * t.nplanes;
+        for (int i = tid; i < t.nplanes; i += COL_THREADS) dst[i] = (float)((double)(long long)col[i] * down);
+    }
 }
 
 // [nsino][nplanes] -> [nplanes][nsino]; 64 x 64 tiles through LDS, 256 threads (4 rows each)
@@ -245,9 +258,9 @@ struct hrrt_delays {
     std::vector<int> h_lor_off;
     int4 *d_lor = nullptr;
     int *d_lor_off = nullptr;
-    int4 *d_seg = nullptr;
+    int2 *d_seg = nullptr;
     float2 *d_z = nullptr;
-    float *d_sings = nullptr;                // transposed singles [layer][head][x][ring], then max |singles|
+    float *d_sings = nullptr;                // transposed singles [layer][head][x][ring], then column maxima
     float *d_cols = nullptr;                 // [nsino][nplanes] columns
     int lds_bytes = 0;
 
@@ -440,6 +453,7 @@ int group_lors(hrrt_delays &c) {
     }
     c.lds_bytes = (int)sizeof(unsigned long long) * c.nplanes;
     if (c.lds_bytes + 1024 > 160 * 1024) return fail_d(HRRT_ERR_UNSUPPORTED, "sinogram column exceeds LDS");
+    if (c.nsino % 8) return fail_d(HRRT_ERR_UNSUPPORTED, "sinogram bins not a multiple of 8");
     return HRRT_OK;
 }
 
@@ -448,26 +462,32 @@ int upload(hrrt_delays &c) {
     if (c.d_lor) return HRRT_OK;
     const std::vector<int4> &lor = c.h_lor;
     const std::vector<int> &count = c.h_lor_off;
-    std::vector<int4> seg(64);
-    for (int i = 0; i < 64; i++) {
-        if (i < c.nsegs) seg[i] = make_int4(c.segz0[i], c.segzmax[i], c.segoff[i], 0);
-        else seg[i] = make_int4(1 << 30, -1, 0, 0);     // no plane: the map's -1 rows
+    // segment entries by e = 2*(int)seg + (seg < 0) + 64 (the kernel's index): e - 64 even and
+    // >= 0 is code e - 64, odd and <= 1 is code 2 - (e - 64); {first plane, last plane, offset}
+    std::vector<int2> seg(128, make_int2(0x7fff, 0));            // first 32767 > last 0: no plane
+    for (int e = 0; e < 128; e++) {
+        const int ee = e - 64;
+        const int code = (ee >= 0 && ee % 2 == 0) ? ee : ((ee <= 1 && (ee & 1)) ? 2 - ee : -1);
+        if (code >= 0 && code < c.nsegs) seg[e] = make_int2(c.segz0[code] | (c.segzmax[code] << 16), c.segoff[code]);
     }
     std::vector<float2> z(NY);
     for (int i = 0; i < NY; i++) z[i] = make_float2(c.zpos[i], c.zpos2[i]);
     DTRY(hipGetDevice(&c.device));
     DTRY(hipMalloc(&c.d_lor, sizeof(int4) * lor.size()));
     DTRY(hipMalloc(&c.d_lor_off, sizeof(int) * count.size()));
-    DTRY(hipMalloc(&c.d_seg, sizeof(int4) * 64));
+    DTRY(hipMalloc(&c.d_seg, sizeof(int2) * 128));
     DTRY(hipMalloc(&c.d_z, sizeof(float2) * NY));
-    DTRY(hipMalloc(&c.d_sings, sizeof(float) * (NCRYS + 1)));
+    DTRY(hipMalloc(&c.d_sings, sizeof(float) * (NCRYS + NCOLS)));
     DTRY(hipMemcpy(c.d_lor, lor.data(), sizeof(int4) * lor.size(), hipMemcpyHostToDevice));
     DTRY(hipMemcpy(c.d_lor_off, count.data(), sizeof(int) * count.size(), hipMemcpyHostToDevice));
-    DTRY(hipMemcpy(c.d_seg, seg.data(), sizeof(int4) * 64, hipMemcpyHostToDevice));
+    DTRY(hipMemcpy(c.d_seg, seg.data(), sizeof(int2) * 128, hipMemcpyHostToDevice));
     DTRY(hipMemcpy(c.d_z, z.data(), sizeof(float2) * NY, hipMemcpyHostToDevice));
-    if (c.lds_bytes > 64 * 1024)
-        DTRY(hipFuncSetAttribute((const void *)delays_column_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (c.lds_bytes > 64 * 1024) {
+        DTRY(hipFuncSetAttribute((const void *)delays_column_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  c.lds_bytes));
+        DTRY(hipFuncSetAttribute((const void *)delays_column_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 c.lds_bytes));
+    }
     return HRRT_OK;
 }
 
@@ -615,13 +635,18 @@ extern "C" int hrrt_delays_smooth(hrrt_delays *c, const float *d_singles, float 
     if (r != HRRT_OK) return r;
     DTRY(hipGetDevice(&dev));
     if (dev != c->device) return fail_d(HRRT_ERR_ARG, "context tables live on another device");
-    if (!c->d_cols) DTRY(hipMalloc(&c->d_cols, sizeof(float) * (size_t)c->nsino * c->nplanes));
     const float ftime = scan_duration > 0.0f ? scan_duration : 1.0f;
     const float tauftime = c->tau * ftime;
     Tables t{c->d_lor, c->d_lor_off, c->d_seg, c->d_z, c->nsino, c->nplanes, c->maxrd + 5};
     sings_prepare_kernel<<<1, PREP_THREADS, 0, s>>>(d_singles, c->d_sings);
     DTRY(hipGetLastError());
-    delays_column_kernel<<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, c->d_cols);
+    if (c->nplanes <= DIRECT_PLANES) {
+        delays_column_kernel<true><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, d_delays);
+        DTRY(hipGetLastError());
+        return HRRT_OK;
+    }
+    if (!c->d_cols) DTRY(hipMalloc(&c->d_cols, sizeof(float) * (size_t)c->nsino * c->nplanes));
+    delays_column_kernel<false><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, c->d_cols);
     DTRY(hipGetLastError());
     column_transpose_kernel<<<dim3((c->nplanes + 63) / 64, (c->nsino + 63) / 64), 256, 0, s>>>(c->d_cols, d_delays,
                                                                                                c->nsino, c->nplanes);

@@ -2,13 +2,13 @@
 
 The kernel computes every crystal-pair term with the reference's float operations (so each
 term equals the oracle's bit for bit) but sums the terms of one sinogram element exactly, in
-64-bit fixed point with unit 2^-k <= M 2^-35 (M = the largest possible term), then rounds
-once to float; the oracle adds in float in the reference's loop order.  All terms are
-positive, so the oracle's sum of n terms is within (n - 1) 2^-24 of the exact sum, the
-kernel's within 2^-24 plus n 2^-(k+1); with the singles below (terms >= M/100) the
-tolerance of 2e-5 relative per element covers n up to 300 terms.  Element totals are
-compared to 1e-6, the kernel's result must be bit-identical from run to run, and every
-element the oracle leaves at zero must be zero.
+64-bit fixed point, and rounds once to float; the oracle adds in float in the reference's
+loop order.  The fixed-point unit of a sinogram bin is at most M nlor 2^-50 (M = the
+largest term the bin's nlor LORs can produce), so an element of n terms is within
+n M nlor 2^-51 of the exact sum; the oracle's float sum is within (n - 1) 2^-24 of it
+(positive terms).  The tolerance of 2e-5 relative per element covers both for n up to 300
+and terms down to 1e-7 M.  Element totals are compared to 1e-6, the kernel's result must be
+bit-identical from run to run, and every element the oracle leaves at zero must be zero.
 """
 import numpy as np
 import pytest
