@@ -158,8 +158,8 @@ def psf_leg(torch, g, image0, normfac_psf, prompt, add, order, steps):
             "note": "two convolve3d per subset (image, correction), sensitivity blurred before inversion"}
 
 
-def randoms_leg(torch, steps, seed, cpu_mps):
-    """Smoothed randoms (gen_delays) for the same span-3 sinogram: singles from a synthetic
+def randoms_leg(torch, steps, seed, cpu_mps, span):
+    """Smoothed randoms (gen_delays) for the workload's sinogram: singles from a synthetic
     coincidence histogram (delayed counts of the model d = tau s F T) on the host, then the
     smoothing on the GPU.  CPU baseline: the oracle on the listed module pairs."""
     import numpy as np
@@ -171,7 +171,7 @@ def randoms_leg(torch, steps, seed, cpu_mps):
     fan = np.array([sum(head[(h + j + 2) % 8] for j in range(5)) for h in range(8)])
     dcoins = rng.poisson(tau * s * np.broadcast_to(fan[None, None, :, None], (2, 104, 8, 72)).reshape(-1) * T)
     coins = np.concatenate([dcoins * 10, dcoins]).astype(np.int32)   # prompts (unused), delayed
-    g = GenDelays(span=GEOM["span"], maxrd=GEOM["maxdel"])
+    g = GenDelays(span=span, maxrd=GEOM["maxdel"])
     t0 = time.perf_counter()
     singles, niter = g.singles(coins, T)
     singles_ms = 1e3 * (time.perf_counter() - t0)
@@ -196,7 +196,7 @@ def randoms_leg(torch, steps, seed, cpu_mps):
     torch.cuda.empty_cache()
     if cpu_mps:
         from oracle import oracle as O
-        D = O.Delays(span=GEOM["span"], maxrd=GEOM["maxdel"])
+        D = O.Delays(span=span, maxrd=GEOM["maxdel"])
         t0 = time.perf_counter()
         _, pairs = D.smooth(singles, tau, T, mps=cpu_mps)
         dt = time.perf_counter() - t0
@@ -318,7 +318,9 @@ def main():
     ap.add_argument("--psf-steps", type=int, default=16,
                     help="timed subset updates with the PSF resolution model (-B, HRRT defaults; 0 = skip)")
     ap.add_argument("--randoms-steps", type=int, default=10,
-                    help="timed smoothed-randoms (gen_delays) computations for the same span (0 = skip)")
+                    help="timed smoothed-randoms (gen_delays) computations (0 = skip)")
+    ap.add_argument("--randoms-span", type=int, default=GEOM["span"], choices=(3, 9),
+                    help="span of the smoothed randoms (default: the workload's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -407,7 +409,8 @@ def main():
         psf = psf_leg(torch, g, image0, normfac_psf, prompt, add, order, args.psf_steps)
     randoms = None
     if args.randoms_steps > 0:
-        randoms = randoms_leg(torch, args.randoms_steps, args.seed + 11, [1, 3] if args.cpu_subsets > 0 else None)
+        randoms = randoms_leg(torch, args.randoms_steps, args.seed + 11, [1, 3] if args.cpu_subsets > 0 else None,
+                              args.randoms_span)
     batched = None
     if args.frames > 0:
         batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
