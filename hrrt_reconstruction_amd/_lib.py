@@ -59,6 +59,23 @@ class DelaysParams(C.Structure):
                 ("diam", C.c_float), ("thick", C.c_float), ("tau", C.c_float), ("lut_file", C.c_char_p)]
 
 
+LM_NBLOCKS = 936
+
+
+class LmOptions(C.Structure):
+    """hrrt_lm_options (list-mode histogramming)."""
+    _fields_ = [("mode", C.c_int), ("start_ms", C.c_int), ("end_ms", C.c_int), ("ignore_border", C.c_int),
+                ("no_doi", C.c_int)]
+
+
+class LmStats(C.Structure):
+    """hrrt_lm_stats."""
+    _fields_ = [(n, C.c_longlong) for n in ("prompts", "randoms", "events", "tags", "time_tags", "rejected",
+                                            "transmission", "first_word", "end_word")] + [
+        ("last_time_ms", C.c_int), ("block_total", C.c_longlong * LM_NBLOCKS),
+        ("block_nsamples", C.c_int * LM_NBLOCKS), ("block_last", C.c_int * LM_NBLOCKS)]
+
+
 SIGNATURES = {
     "hrrt_create": (_I, [C.POINTER(Params), C.POINTER(_VP)]),
     "hrrt_destroy": (None, [_VP]),
@@ -115,6 +132,9 @@ SIGNATURES = {
     "hrrt_delays_smooth": (_I, [_VP, _VP, C.c_float, _VP, _VP]),
     "hrrt_gen_delays": (_I, [_VP, _VP, C.c_float, _VP, _VP, _VP, _VP]),
     "hrrt_delays_write": (_I, [_VP, _VP, C.c_char_p, _VP]),
+    # include/hrrt_lmhist.h
+    "hrrt_lm_histogram": (_I, [_VP, _VP, C.c_longlong, C.POINTER(LmOptions), _VP, _VP, _VP, _VP,
+                               C.POINTER(LmStats), _VP]),
 }
 
 _lib = None

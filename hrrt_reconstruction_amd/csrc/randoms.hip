@@ -22,31 +22,12 @@
 #include <vector>
 
 #include "../../include/hrrt_randoms.h"
+#include "delays.h"
 #include "kernels.h"
 
+using namespace hrrt_dl;
+
 namespace {
-
-constexpr int NHEADS = 8, NDOIS = 2, NX = 72, NY = 104;
-constexpr int NXD = NX * NDOIS;                   // 144 crystals across a head, both layers
-constexpr int NCRYS = NHEADS * NDOIS * NX * NY;   // 119808
-constexpr int XYH = NX * NY * NHEADS;             // one layer, [ring][head][x]
-constexpr int HX = NHEADS * NX;                   // one ring over the heads
-constexpr int NMP = 20;
-constexpr int SEGMAP = 63;                        // segment rows of the plane map (lor_sinogram_map.cpp:154-158)
-
-// HRRT_MPAIRS geometry_info.h:68-73
-constexpr int kMpairs[NMP + 1][2] = {{-1, -1}, {0, 2}, {0, 3}, {0, 4}, {0, 5}, {0, 6}, {1, 3}, {1, 4},
-                                     {1, 5},   {1, 6}, {1, 7}, {2, 4}, {2, 5}, {2, 6}, {2, 7}, {3, 5},
-                                     {3, 6},   {3, 7}, {4, 6}, {4, 7}, {5, 7}};
-// span-3 segment -> span-9 segment (conversiontable, segment_info_fromarchive.cpp:23-26)
-constexpr int kS3toS9[45] = {0, 0, 0, 1, 2, 1, 2, 1, 2, 3, 4, 3, 4, 3, 4, 5, 6, 5, 6, 5, 6, 7, 8,
-                             7, 8, 7, 8, 9, 10, 9, 10, 9, 10, 11, 12, 11, 12, 11, 12, 13, 14, 13, 14, 13, 14};
-
-struct Sol {       // SOL lor_sinogram_map.h:18-22 (also the LUT file record)
-    int nsino;
-    float d, z;
-};
-static_assert(sizeof(Sol) == 12, "LUT record is 12 bytes");
 
 struct Tables {    // device tables of one context
     const int4 *lor;      // [nlor] {A ring-column offset, B ring-column offset, bits(d), bits(z)}
@@ -241,88 +222,6 @@ __global__ __launch_bounds__(256) void column_transpose_kernel(const float *__re
 
 }  // namespace
 
-// ---------------------------------------------------------------- host context
-
-struct hrrt_delays {
-    int span = 9, maxrd = 67, lr_type = 0;
-    int nprojs = 256, nviews = 288, nsino = 0, nplanes = 0, nsegs = 0;
-    float tau = 6e-9f, depth = 1.0f;
-    double binsize = 0, plane_sep = 0, crystal_radius = 0, d_tan_theta = 0;
-    double sin_head[NHEADS]{}, cos_head[NHEADS]{};
-    std::vector<Sol> lut;                    // [21][144][144], module pair 0 unused
-    float zpos[NY]{}, zpos2[NY]{};
-    int segz0[SEGMAP]{}, segzmax[SEGMAP]{}, segoff[SEGMAP]{};
-    long long pairs_full = 0;
-    int nlor = 0, device = 0;
-    std::vector<int4> h_lor;                 // LORs grouped by bin (device copy d_lor)
-    std::vector<int> h_lor_off;
-    int4 *d_lor = nullptr;
-    int *d_lor_off = nullptr;
-    int2 *d_seg = nullptr;
-    float2 *d_z = nullptr;
-    float *d_sings = nullptr;                // transposed singles [layer][head][x][ring], then column maxima
-    float *d_cols = nullptr;                 // [nsino][nplanes] columns
-    int lds_bytes = 0;
-
-    // calc_det_to_phy geometry_info.cpp:137-163: block/crystal offsets in float, the head
-    // rotation in double, stored as float (constants geometry_info.h:51-55 are floats)
-    void crystal(int head, int layer, int detx, int dety, float pos[3]) const {
-        const float csize = 0.22f, cgap = 0.02f, bgap = 0.05f;
-        const float bsize = (float)(8.0 * (double)csize + (double)(7.0f * cgap));
-        const float xhsize = (float)(9.0 * (double)bsize + (double)(8.0f * bgap));
-        const float bx = (float)(detx / 8) * (bsize + bgap) + (float)(detx % 8) * (csize + cgap);
-        const double x = (double)bx - (double)xhsize / 2.0 + (double)csize / 2.0;
-        const double y = crystal_radius + (double)(depth * (float)(1 - layer)) + 0.5 * (double)depth;
-        const float bz = (float)(dety / 8) * (bsize + bgap) + (float)(dety % 8) * (csize + cgap);
-        const double s = sin_head[head], c = cos_head[head];
-        pos[0] = (float)(x * c + y * s);
-        pos[1] = (float)(-x * s + y * c);
-        pos[2] = bz;
-    }
-
-    // init_phy_to_pro lor_sinogram_map.cpp:99-138
-    Sol lor(const float a[3], const float b[3]) const {
-        const double dy = (double)(a[1] - b[1]), dx = (double)(a[0] - b[0]);
-        double d = std::sqrt(dx * dx + dy * dy);
-        float r = (float)((double)(a[1] * b[0] - a[0] * b[1]) / d);
-        float phi = (float)std::atan2(dx, dy);
-        if (phi < 0.0f) {
-            phi = phi + (float)M_PI;
-            r = -r;
-            d = -d;
-        }
-        if ((double)phi >= M_PI) {
-            phi = 0.0f;
-            r = -r;
-            d = -d;
-        }
-        int view = (int)((double)((float)nviews * phi) / M_PI);
-        if (view >= 288 || view < 0) view = 0;
-        const float z = (float)(((double)a[0] * dx + (double)a[1] * dy) / (d * d));
-        const int bin = (int)((double)(nprojs / 2) + ((double)r / binsize + 0.5));
-        Sol s;
-        s.nsino = (bin < 0 || bin > nprojs - 1) ? -1 : bin + view * nprojs;
-        s.d = (float)(1.0 / d / d_tan_theta);
-        s.z = (float)((double)z / plane_sep);
-        return s;
-    }
-
-    // segment sizes of one span (init_seginfo segment_info_fromarchive.cpp:35-78): first
-    // plane, last plane and file offset per segment; returns the plane total
-    int segments(int sp, int *z0, int *zmax, int *off) const {
-        const int n = 2 * (maxrd / sp) + 1, np = 2 * NY - 1, half = (sp + 1) / 2;
-        int total = 0;
-        for (int i = 0; i < n; i++) {
-            z0[i] = i == 0 ? 0 : half + sp * ((i - 1) / 2);
-            const int nz = np - 2 * z0[i];
-            zmax[i] = z0[i] + nz - 1;
-            off[i] = total - z0[i];
-            total += nz;
-        }
-        return total;
-    }
-};
-
 namespace {
 
 int make_tables(hrrt_delays &c, const hrrt_delays_params &p, std::string &err) {
@@ -406,13 +305,7 @@ int read_lut(hrrt_delays &c, const char *path, std::string &err) {
     return HRRT_OK;
 }
 
-int fail_d(int code, const std::string &msg) { return hrrt::set_last_error(code, msg.c_str()); }
 
-#define DTRY(expr)                                                                                        \
-    do {                                                                                                  \
-        hipError_t e_ = (expr);                                                                           \
-        if (e_ != hipSuccess) return fail_d(HRRT_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
 
 // Group the valid LORs by sinogram bin (host tables; uploaded by the first smoothing).
 int group_lors(hrrt_delays &c) {
@@ -422,7 +315,7 @@ int group_lors(hrrt_delays &c) {
         for (int i = 0; i < NXD * NXD; i++) {
             const Sol &s = c.lut[(size_t)mp * NXD * NXD + i];
             if (s.nsino < -1 || s.nsino >= c.nsino)
-                return fail_d(HRRT_ERR_ARG, "LUT bin outside the sinogram (LUT built for another LR type?)");
+                return hrrt_dl::fail(HRRT_ERR_ARG, "LUT bin outside the sinogram (LUT built for another LR type?)");
             if (s.nsino >= 0) count[s.nsino + 1]++;
         }
     for (int b = 0; b < c.nsino; b++) count[b + 1] += count[b];
@@ -452,13 +345,35 @@ int group_lors(hrrt_delays &c) {
             }
     }
     c.lds_bytes = (int)sizeof(unsigned long long) * c.nplanes;
-    if (c.lds_bytes + 1024 > 160 * 1024) return fail_d(HRRT_ERR_UNSUPPORTED, "sinogram column exceeds LDS");
-    if (c.nsino % 8) return fail_d(HRRT_ERR_UNSUPPORTED, "sinogram bins not a multiple of 8");
+    if (c.lds_bytes + 1024 > 160 * 1024) return hrrt_dl::fail(HRRT_ERR_UNSUPPORTED, "sinogram column exceeds LDS");
+    if (c.nsino % 8) return hrrt_dl::fail(HRRT_ERR_UNSUPPORTED, "sinogram bins not a multiple of 8");
     return HRRT_OK;
 }
 
+// Head totals of the singles (compute_drate gen_delays_lib.cpp:168-176): layer, x, ring order.
+void head_sums(const float *srate, float *hsum) {
+    for (int head = 0; head < NHEADS; head++) {
+        float s = 0.0f;
+        for (int layer = 0; layer < NDOIS; layer++)
+            for (int cx = 0; cx < NX; cx++)
+                for (int cy = 0; cy < NY; cy++) s += srate[NX * head + cx + HX * cy + XYH * layer];
+        hsum[head] = s;
+    }
+}
+
+// Sum over the five heads in coincidence with `head` (gen_delays_lib.cpp:180-186).
+float fan_sum(const float *hsum, int head) {
+    float s = 0.0f;
+    for (int j = 0; j < 5; j++) s += hsum[(head + j + 2) % NHEADS];
+    return s;
+}
+
+}  // namespace
+
+int hrrt_dl::fail(int code, const std::string &msg) { return hrrt::set_last_error(code, msg.c_str()); }
+
 // Device copies of the tables, on the current device, at the first smoothing.
-int upload(hrrt_delays &c) {
+int hrrt_dl::upload(hrrt_delays &c) {
     if (c.d_lor) return HRRT_OK;
     const std::vector<int4> &lor = c.h_lor;
     const std::vector<int> &count = c.h_lor_off;
@@ -491,33 +406,13 @@ int upload(hrrt_delays &c) {
     return HRRT_OK;
 }
 
-// Head totals of the singles (compute_drate gen_delays_lib.cpp:168-176): layer, x, ring order.
-void head_sums(const float *srate, float *hsum) {
-    for (int head = 0; head < NHEADS; head++) {
-        float s = 0.0f;
-        for (int layer = 0; layer < NDOIS; layer++)
-            for (int cx = 0; cx < NX; cx++)
-                for (int cy = 0; cy < NY; cy++) s += srate[NX * head + cx + HX * cy + XYH * layer];
-        hsum[head] = s;
-    }
-}
-
-// Sum over the five heads in coincidence with `head` (gen_delays_lib.cpp:180-186).
-float fan_sum(const float *hsum, int head) {
-    float s = 0.0f;
-    for (int j = 0; j < 5; j++) s += hsum[(head + j + 2) % NHEADS];
-    return s;
-}
-
-}  // namespace
-
 // ---------------------------------------------------------------- C ABI
 
 extern "C" int hrrt_delays_create(const hrrt_delays_params *params, hrrt_delays **out) {
-    if (!params || !out) return fail_d(HRRT_ERR_ARG, "null argument");
+    if (!params || !out) return hrrt_dl::fail(HRRT_ERR_ARG, "null argument");
     *out = nullptr;
     hrrt_delays *c = new (std::nothrow) hrrt_delays();
-    if (!c) return fail_d(HRRT_ERR_NOMEM, "host allocation failed");
+    if (!c) return hrrt_dl::fail(HRRT_ERR_NOMEM, "host allocation failed");
     std::string err;
     int r = make_tables(*c, *params, err);
     if (r == HRRT_OK) {
@@ -526,7 +421,7 @@ extern "C" int hrrt_delays_create(const hrrt_delays_params *params, hrrt_delays 
     }
     if (r != HRRT_OK) {
         delete c;
-        return fail_d(r, err);
+        return hrrt_dl::fail(r, err);
     }
     if ((r = group_lors(*c)) != HRRT_OK) {
         delete c;
@@ -539,14 +434,15 @@ extern "C" int hrrt_delays_create(const hrrt_delays_params *params, hrrt_delays 
 extern "C" void hrrt_delays_destroy(hrrt_delays *c) {
     if (!c) return;
     for (void *p : {(void *)c->d_lor, (void *)c->d_lor_off, (void *)c->d_seg, (void *)c->d_z, (void *)c->d_sings,
-                    (void *)c->d_cols})
+                    (void *)c->d_cols, (void *)c->d_sol, (void *)c->d_lm_counts, (void *)c->d_lm_chunks,
+                    (void *)c->d_lm_stats})
         if (p) (void)hipFree(p);
     delete c;
 }
 
 extern "C" int hrrt_delays_get_dims(const hrrt_delays *c, int *nprojs, int *nviews, int *nplanes, int *nsegs,
                                     long long *pairs_full) {
-    if (!c) return fail_d(HRRT_ERR_ARG, "null context");
+    if (!c) return hrrt_dl::fail(HRRT_ERR_ARG, "null context");
     if (nprojs) *nprojs = c->nprojs;
     if (nviews) *nviews = c->nviews;
     if (nplanes) *nplanes = c->nplanes;
@@ -556,8 +452,8 @@ extern "C" int hrrt_delays_get_dims(const hrrt_delays *c, int *nprojs, int *nvie
 }
 
 extern "C" int hrrt_delays_get_lut(const hrrt_delays *c, int mp, int32_t *nsino, float *d, float *z) {
-    if (!c) return fail_d(HRRT_ERR_ARG, "null context");
-    if (mp < 1 || mp > NMP) return fail_d(HRRT_ERR_ARG, "module pair must be 1..20");
+    if (!c) return hrrt_dl::fail(HRRT_ERR_ARG, "null context");
+    if (mp < 1 || mp > NMP) return hrrt_dl::fail(HRRT_ERR_ARG, "module pair must be 1..20");
     const Sol *s = &c->lut[(size_t)mp * NXD * NXD];
     for (int i = 0; i < NXD * NXD; i++) {
         if (nsino) nsino[i] = s[i].nsino;
@@ -569,14 +465,14 @@ extern "C" int hrrt_delays_get_lut(const hrrt_delays *c, int mp, int32_t *nsino,
 
 // save_lut_sol lor_sinogram_map.cpp:305-340
 extern "C" int hrrt_delays_save_lut(const hrrt_delays *c, const char *path) {
-    if (!c || !path) return fail_d(HRRT_ERR_ARG, "null argument");
+    if (!c || !path) return hrrt_dl::fail(HRRT_ERR_ARG, "null argument");
     FILE *f = fopen(path, "wb");
-    if (!f) return fail_d(HRRT_ERR_ARG, std::string("cannot create ") + path);
+    if (!f) return hrrt_dl::fail(HRRT_ERR_ARG, std::string("cannot create ") + path);
     const size_t n = (size_t)NMP * NXD * NXD;
     bool ok = fwrite(&c->lut[(size_t)NXD * NXD], sizeof(Sol), n, f) == n;
     ok = ok && fwrite(c->zpos, sizeof(float), NY, f) == (size_t)NY && fwrite(c->zpos2, sizeof(float), NY, f) == (size_t)NY;
     ok = (fclose(f) == 0) && ok;
-    return ok ? HRRT_OK : fail_d(HRRT_ERR_ARG, std::string("write failed: ") + path);
+    return ok ? HRRT_OK : hrrt_dl::fail(HRRT_ERR_ARG, std::string("write failed: ") + path);
 }
 
 // compute_csings_from_drates gen_delays_lib.cpp:254-271 with compute_initial_srate (:233-241),
@@ -584,7 +480,7 @@ extern "C" int hrrt_delays_save_lut(const hrrt_delays *c, const char *path) {
 // sum is 64-bit (the reference's int overflows on large errors; only its zero test matters).
 extern "C" int hrrt_delays_singles(const hrrt_delays *c, const int32_t *coins, float scan_duration, float *singles,
                                    int *niter) {
-    if (!c || !coins || !singles) return fail_d(HRRT_ERR_ARG, "null argument");
+    if (!c || !coins || !singles) return hrrt_dl::fail(HRRT_ERR_ARG, "null argument");
     const float dt = scan_duration > 0.0f ? scan_duration : 1.0f, tau = c->tau;
     const int32_t *dcoins = coins + NCRYS;
     std::vector<float> drates(NCRYS), xrates(NCRYS);
@@ -629,12 +525,12 @@ extern "C" int hrrt_delays_singles(const hrrt_delays *c, const int32_t *coins, f
 
 extern "C" int hrrt_delays_smooth(hrrt_delays *c, const float *d_singles, float scan_duration, float *d_delays,
                                   void *stream) {
-    if (!c || !d_singles || !d_delays) return fail_d(HRRT_ERR_ARG, "null argument");
+    if (!c || !d_singles || !d_delays) return hrrt_dl::fail(HRRT_ERR_ARG, "null argument");
     hipStream_t s = (hipStream_t)stream;
     int r = upload(*c), dev = -1;
     if (r != HRRT_OK) return r;
     DTRY(hipGetDevice(&dev));
-    if (dev != c->device) return fail_d(HRRT_ERR_ARG, "context tables live on another device");
+    if (dev != c->device) return hrrt_dl::fail(HRRT_ERR_ARG, "context tables live on another device");
     const float ftime = scan_duration > 0.0f ? scan_duration : 1.0f;
     const float tauftime = c->tau * ftime;
     Tables t{c->d_lor, c->d_lor_off, c->d_seg, c->d_z, c->nsino, c->nplanes, c->maxrd + 5};
@@ -656,7 +552,7 @@ extern "C" int hrrt_delays_smooth(hrrt_delays *c, const float *d_singles, float 
 
 extern "C" int hrrt_gen_delays(hrrt_delays *c, const int32_t *coins, float scan_duration, float *d_delays,
                                float *singles_out, int *niter, void *stream) {
-    if (!c || !coins || !d_delays) return fail_d(HRRT_ERR_ARG, "null argument");
+    if (!c || !coins || !d_delays) return hrrt_dl::fail(HRRT_ERR_ARG, "null argument");
     std::vector<float> sg(NCRYS);
     int r = hrrt_delays_singles(c, coins, scan_duration, sg.data(), niter);
     if (r != HRRT_OK) return r;
@@ -670,12 +566,12 @@ extern "C" int hrrt_gen_delays(hrrt_delays *c, const int32_t *coins, float scan_
 }
 
 extern "C" int hrrt_delays_write(hrrt_delays *c, const float *d_delays, const char *path, void *stream) {
-    if (!c || !d_delays || !path) return fail_d(HRRT_ERR_ARG, "null argument");
+    if (!c || !d_delays || !path) return hrrt_dl::fail(HRRT_ERR_ARG, "null argument");
     hipStream_t s = (hipStream_t)stream;
     const size_t plane = (size_t)c->nsino;
     std::vector<float> h(plane * 64);
     FILE *f = fopen(path, "wb");
-    if (!f) return fail_d(HRRT_ERR_ARG, std::string("cannot create ") + path);
+    if (!f) return hrrt_dl::fail(HRRT_ERR_ARG, std::string("cannot create ") + path);
     bool ok = true;
     for (int p0 = 0; ok && p0 < c->nplanes; p0 += 64) {
         const size_t n = plane * std::min(64, c->nplanes - p0);
@@ -683,10 +579,10 @@ extern "C" int hrrt_delays_write(hrrt_delays *c, const float *d_delays, const ch
                 hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess) {
             fclose(f);
-            return fail_d(HRRT_ERR_HIP, "copy of the delays to the host failed");
+            return hrrt_dl::fail(HRRT_ERR_HIP, "copy of the delays to the host failed");
         }
         ok = fwrite(h.data(), sizeof(float), n, f) == n;
     }
     ok = (fclose(f) == 0) && ok;
-    return ok ? HRRT_OK : fail_d(HRRT_ERR_ARG, std::string("write failed: ") + path);
+    return ok ? HRRT_OK : hrrt_dl::fail(HRRT_ERR_ARG, std::string("write failed: ") + path);
 }

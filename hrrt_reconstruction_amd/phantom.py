@@ -163,3 +163,75 @@ def dynamic_frames(nframes, counts_lo, counts_hi, gen):
     r = torch.rand(6, generator=gen)
     rates = [float(math.exp(math.log(0.005) + (math.log(0.2) - math.log(0.005)) * float(v))) for v in r]
     return dur, mids, counts, rates
+
+
+# ---------------------------------------------------------------- list mode
+
+def lm_event_words(mp, ax, ay, bx, by, doia, doib, prompt):
+    """64-bit HRRT list-mode events as (ew1, ew2) uint32 words (numpy arrays): the fields
+    rebin_packet decodes (lmhistogram_mp/histogram_mp.cpp:1120-1128); the top bits make the
+    pair an event (EWTYPES, gen_delays_lib/geometry_info.h:75): ew2 top bits 11 = prompt,
+    10 = delayed."""
+    import numpy as np
+    u = lambda v: np.asarray(v, dtype=np.uint64)
+    ew1 = (u(doia) << 22) | ((u(mp) & 7) << 16) | (u(ay) << 8) | u(ax)
+    ew2 = np.where(np.asarray(prompt, bool), np.uint64(0xC0000000), np.uint64(0x80000000))
+    ew2 = ew2 | (u(doib) << 22) | (((u(mp) >> 3) & 7) << 16) | (u(by) << 8) | u(bx)
+    return ew1.astype(np.uint32), ew2.astype(np.uint32)
+
+
+def lm_tag_words(tag):
+    """A 32-bit tag word as an (ew1, ew2) pair: low half in ew1 (top bits 01), high half in ew2
+    (top bits 10)."""
+    import numpy as np
+    tag = np.asarray(tag, dtype=np.uint64)
+    return ((np.uint64(0x40000000) | (tag & np.uint64(0xffff))).astype(np.uint32),
+            (np.uint64(0x80000000) | (tag >> np.uint64(16))).astype(np.uint32))
+
+
+def lm_synthetic_stream(nevents, seed, events_per_ms=2000, t0_ms=1000, delayed_fraction=0.1, stray_every=0,
+                        bad_fraction=0.0, tx_fraction=0.0, odd_doi_fraction=0.0, singles_every=0):
+    """A seeded synthetic list-mode stream (numpy uint32 words): random emission events of the
+    20 module pairs (rings within 40 of each other), a time tag (ms) every events_per_ms
+    events from t0_ms, optional block-singles tags, stray out-of-sync words, rejected events
+    (bad module pair or crystal), transmission (DOI 7) and DOI 2-6 events."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = int(nevents)
+    mp = rng.integers(1, 21, n)
+    ax, bx = rng.integers(0, 72, n), rng.integers(0, 72, n)
+    ay = rng.integers(0, 104, n)
+    by = np.clip(ay + rng.integers(-40, 41, n), 0, 103)
+    doia, doib = rng.integers(0, 2, n), rng.integers(0, 2, n)
+    prompt = rng.random(n) >= delayed_fraction
+    if bad_fraction > 0:
+        bad = rng.random(n) < bad_fraction
+        kind = rng.integers(0, 3, n)
+        mp = np.where(bad & (kind == 0), rng.choice([0, 21, 40, 63], n), mp)
+        ax = np.where(bad & (kind == 1), rng.integers(72, 256, n), ax)
+        by = np.where(bad & (kind == 2), rng.integers(104, 256, n), by)
+    if tx_fraction > 0:
+        doia = np.where(rng.random(n) < tx_fraction, 7, doia)
+    if odd_doi_fraction > 0:
+        doib = np.where(rng.random(n) < odd_doi_fraction, rng.integers(2, 7, n), doib)
+    ew1, ew2 = lm_event_words(mp, ax, ay, bx, by, doia, doib, prompt)
+    pairs = np.stack([ew1, ew2], axis=1)
+    # tags: time every events_per_ms events, singles in between
+    chunks = []
+    t = t0_ms
+    for i0 in range(0, n, events_per_ms):
+        tw = np.stack(lm_tag_words(np.array([0x80000000 | t])), axis=1)
+        chunks.append(tw)
+        if singles_every:
+            nb = max(1, events_per_ms // singles_every)
+            blocks = rng.integers(0, 936, nb)
+            vals = rng.integers(0, 1 << 19, nb)
+            chunks.append(np.stack(lm_tag_words((0xA0000000 | (blocks << 19) | vals).astype(np.uint64)), axis=1))
+        chunks.append(pairs[i0:i0 + events_per_ms])
+        t += 1
+    words = np.concatenate(chunks, axis=0).reshape(-1)
+    if stray_every:
+        pos = np.sort(rng.choice(words.size, max(1, words.size // stray_every), replace=False))
+        stray = (0x80000000 | rng.integers(0, 1 << 30, pos.size)).astype(np.uint32)
+        words = np.insert(words, pos, stray)
+    return words.astype(np.uint32)

@@ -215,6 +215,13 @@ int       og_delays_singles(const int *dcoins, float tau, float dt, float *srate
 long long og_delays_smooth(const og_delays *g, const float *csings, float tau, float ftime, const int *mps, int nmp,
                            float *out);
 
+/* ---- list-mode histogramming, lmhist.c ---- */
+long long og_lm_rebin(const og_delays *g, int mp, int alayer, int ax, int ay, int blayer, int bx, int by);
+void      og_lm_histogram(const og_delays *g, const uint32_t *w, long long nwords, int mode, int start_ms, int end_ms,
+                          int ignore_border, int doi_processing, int16_t *sino, uint8_t *delayed, uint32_t *coin_p,
+                          uint32_t *coin_d, long long *stats, long long *block_total, int *block_nsamples,
+                          int *block_last);
+
 #ifdef __cplusplus
 }
 #endif

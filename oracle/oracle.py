@@ -514,3 +514,46 @@ class Delays:
         m = None if mps is None else np.ascontiguousarray(mps, np.int32)
         pairs = self.L.og_delays_smooth(self.g, _ptr(csings), tau, ftime, _ptr(m), 0 if m is None else m.size, _ptr(out))
         return out, int(pairs)
+
+
+LM_PACKET_WORDS = 2 * 1024 * 1024
+NBLOCKS = 936
+
+
+def _lm_lib():
+    L = _delays_lib()
+    if not getattr(L, "_lm_bound", False):
+        vp = C.c_void_p
+        L.og_lm_rebin.restype = C.c_longlong
+        L.og_lm_rebin.argtypes = [vp] + [C.c_int] * 7
+        L.og_lm_histogram.argtypes = [vp, vp, C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      vp, vp, vp, vp, vp, vp, vp, vp]
+        L._lm_bound = True
+    return L
+
+
+def lm_histogram(D, words, mode=0, start_ms=0, end_ms=0, ignore_border=False, doi=True, delayed_bytes=True):
+    """List-mode histogram of a word stream with the tables of Delays D.
+    Returns dict(sino int16 [nplanes][nviews][nprojs] (x2 planes for mode 1 without a byte array),
+    delayed uint8 or None, coin_p, coin_d uint32 [crystals], stats int64 [10], blocks)."""
+    L = _lm_lib()
+    words = np.ascontiguousarray(words, np.uint32)
+    S = D.nplanes * D.nsino
+    halves = 2 if (mode == 1 and not delayed_bytes) else 1
+    sino = np.zeros(halves * S, np.int16)
+    delayed = np.zeros(S, np.uint8) if (mode == 1 and delayed_bytes) else None
+    coin_p = np.zeros(NCRYS, np.uint32)
+    coin_d = np.zeros(NCRYS, np.uint32)
+    stats = np.zeros(10, np.int64)
+    btot = np.zeros(NBLOCKS, np.int64)
+    bns = np.zeros(NBLOCKS, np.int32)
+    blast = np.zeros(NBLOCKS, np.int32)
+    L.og_lm_histogram(D.g, _ptr(words), words.size, mode, start_ms, end_ms, int(ignore_border), int(doi),
+                      _ptr(sino), _ptr(delayed), _ptr(coin_p), _ptr(coin_d), _ptr(stats), _ptr(btot), _ptr(bns),
+                      _ptr(blast))
+    return dict(sino=sino.reshape(halves * D.nplanes, D.nviews, D.nprojs), delayed=delayed, coin_p=coin_p,
+                coin_d=coin_d, stats=stats, block_total=btot, block_nsamples=bns, block_last=blast)
+
+
+def lm_rebin(D, mp, alayer, ax, ay, blayer, bx, by):
+    return int(_lm_lib().og_lm_rebin(D.g, mp, alayer, ax, ay, blayer, bx, by))

@@ -11,7 +11,7 @@ import pytest
 
 from conftest import GEOMS, ROOT
 
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("hrrt_osem.h", "hrrt_randoms.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("hrrt_osem.h", "hrrt_randoms.h", "hrrt_lmhist.h")]
 
 
 def declared_functions():
