@@ -206,6 +206,45 @@ def randoms_leg(torch, steps, seed, cpu_mps, span):
     return res
 
 
+def listmode_leg(torch, events, steps, seed, cpu_events, span):
+    """List-mode histogramming: a synthetic 64-bit stream (events of the 20 module pairs, a time
+    tag per ms, block singles tags, 10% delayed) histogrammed in trues mode into the workload's
+    sinogram, with the coincidence maps.  CPU baseline: the oracle on the first cpu_events."""
+    import numpy as np
+    from hrrt_reconstruction_amd import lmhist
+    from hrrt_reconstruction_amd.phantom import lm_synthetic_stream
+    from hrrt_reconstruction_amd.randoms import GenDelays
+    words = lm_synthetic_stream(events, seed, events_per_ms=20000, singles_every=2000)
+    d = torch.from_numpy(words.view(np.int32)).cuda()
+    g = GenDelays(span=span, maxrd=GEOM["maxdel"])
+    for _ in range(2):
+        lmhist.histogram(g, d, mode=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sino, _, cp, cd, st = lmhist.histogram(g, d, mode=0)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    log(f"list mode: {events} events in {ms:.2f} ms ({events / ms / 1e6:.2f} G events/s)")
+    res = {"events": events, "words": int(words.size), "ms_per_histogram": round(ms, 3),
+           "events_per_s": round(events / (ms * 1e-3), 1), "prompts_binned": st["prompts"], "span": span,
+           "note": "whole stream per call: zeroing, chunk walk, histogram, int16 finalize, statistics", "cpu_baseline": None}
+    del sino, cp, cd, d
+    torch.cuda.empty_cache()
+    if cpu_events:
+        from oracle import oracle as O
+        D = O.Delays(span=span, maxrd=GEOM["maxdel"])
+        n = min(cpu_events, events)
+        sub = words[: 2 * n + 2 * (n // 20000 + 1) * 1 + 8]
+        t0 = time.perf_counter()
+        r = O.lm_histogram(D, sub, mode=0)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(int(r["stats"][2]) / dt, 1), "unit": "events/s", "cores": 1,
+                               "kind": "port", "sample": f"first {int(r['stats'][2])} events of the stream ({dt:.1f} s)"}
+        log(f"list mode CPU oracle: {int(r['stats'][2]) / dt / 1e6:.1f} M events/s")
+    return res
+
+
 def batched_frames(torch, nframes, iterations, seed):
     """Dynamic study in batched mode: every frame's subsets run in one launch per kernel, each subset
     of an iteration replayed from a HIP graph.  FP32 form and FP16-input precision form."""
@@ -321,6 +360,9 @@ def main():
                     help="timed smoothed-randoms (gen_delays) computations (0 = skip)")
     ap.add_argument("--randoms-span", type=int, default=GEOM["span"], choices=(3, 9),
                     help="span of the smoothed randoms (default: the workload's)")
+    ap.add_argument("--listmode-events", type=int, default=40_000_000,
+                    help="events of the synthetic list-mode stream histogrammed per step (0 = skip)")
+    ap.add_argument("--listmode-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -411,6 +453,10 @@ def main():
     if args.randoms_steps > 0:
         randoms = randoms_leg(torch, args.randoms_steps, args.seed + 11, [1, 3] if args.cpu_subsets > 0 else None,
                               args.randoms_span)
+    listmode = None
+    if args.listmode_events > 0:
+        listmode = listmode_leg(torch, args.listmode_events, args.listmode_steps, args.seed + 13,
+                                20_000_000 if args.cpu_subsets > 0 else 0, GEOM["span"])
     batched = None
     if args.frames > 0:
         batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
@@ -442,6 +488,7 @@ def main():
         "cpu_baseline": cpu,
         "psf": psf,
         "randoms": randoms,
+        "listmode": listmode,
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

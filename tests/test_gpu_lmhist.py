@@ -104,3 +104,41 @@ def test_list_mode_to_smoothed_randoms(torch_dev, stream):
     assert np.array_equal(out == 0, ref_d == 0)
     err = np.abs(out.astype(np.float64) - ref_d) - 2e-5 * np.abs(ref_d.astype(np.float64))
     assert err.max() <= 0
+
+
+def test_list_mode_to_osem_subset(torch_dev, oracle_lib):
+    """The whole device pipeline at C1 (span 9, -X 128): list mode -> prompt sinogram and .ch ->
+    smoothed randoms -> staging -> one OSEM subset, against the oracle staging and subset on
+    the same files."""
+    from conftest import GEOMS
+    from gpu_util import from_dev
+    from hrrt_reconstruction_amd import lmhist
+    from hrrt_reconstruction_amd.osem import Osem3D
+    from hrrt_reconstruction_amd.phantom import lm_synthetic_stream
+    from hrrt_reconstruction_amd.randoms import GenDelays
+    print("list mode -> OSEM subset (c1)", flush=True)
+    words = lm_synthetic_stream(3_000_000, 13, events_per_ms=3000, delayed_fraction=0.2)
+    g = GenDelays(span=9, maxrd=67)
+    prompts, _, cp, cd, st = lmhist.histogram(g, torch_dev.from_numpy(words.view(np.int32)).cuda(), mode=2)
+    assert st["prompts"] > 1_000_000
+    delays, _, _ = g.gen_delays(lmhist.coincidence_histogram(cp, cd), 1.0)
+    rng = np.random.default_rng(14)
+    shape = tuple(delays.shape)
+    norm = rng.uniform(0.8, 1.2, shape).astype(np.float32)
+    atten = np.exp(rng.uniform(0.0, 1.0, shape)).astype(np.float32)
+    G = Osem3D("cuda:0", **GEOMS["c1"])
+    pr, add, inv = G.stage(prompts, delays, torch_dev.from_numpy(norm).cuda(), None,
+                           torch_dev.from_numpy(atten).cuda())
+    o = oracle_lib.Oracle(**GEOMS["c1"])
+    ref_prompt, ref_add = o.prepare_w3(prompts.cpu().numpy(), delays.cpu().numpy(), norm, None, atten)
+    assert np.array_equal(from_dev(pr), G.sino_to_paircol(ref_prompt))
+    assert np.array_equal(from_dev(add), G.sino_to_paircol(ref_add))
+    nf = G.calculate_normfac(inv)
+    image = G.init_image()
+    img_ref = from_dev(image).copy()
+    nf_host = from_dev(nf)
+    s = int(G.order[0])
+    G.osem_subset(image, nf, pr, add, s)
+    o.osem_subset(img_ref, o.image_zeros(), np.ascontiguousarray(nf_host[s]), ref_add, ref_prompt, s,
+                  nthreads=G.backproject_schedule, workers=8)
+    assert np.array_equal(from_dev(image), img_ref)
