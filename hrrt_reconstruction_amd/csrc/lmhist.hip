@@ -151,18 +151,24 @@ __global__ __launch_bounds__(WAVES * 64) void lm_summary_kernel(LmArgs a, long l
     }
 }
 
-// one thread per packet: the walk enters the packet's first chunk at word 0
-__global__ void lm_resolve_kernel(unsigned *chunks, long long nchunks) {
-    const long long pk = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long c0 = pk * CHUNKS_PER_PACKET;
-    if (c0 >= nchunks) return;
-    const long long c1 = c0 + CHUNKS_PER_PACKET < nchunks ? c0 + CHUNKS_PER_PACKET : nchunks;
-    unsigned e = 0;
-    for (long long c = c0; c < c1; c++) {
-        const unsigned s = chunks[c] & 3u;
-        chunks[c] = s | (e << 2);
-        e = (s >> e) & 1u;
+// one workgroup per packet: the chunk summaries staged in LDS, walked in order by one thread
+// (the walk enters the packet's first chunk at word 0)
+__global__ __launch_bounds__(256) void lm_resolve_kernel(unsigned *chunks, long long nchunks) {
+    __shared__ unsigned sum[CHUNKS_PER_PACKET];
+    const long long c0 = (long long)blockIdx.x * CHUNKS_PER_PACKET;
+    const int n = (int)(c0 + CHUNKS_PER_PACKET < nchunks ? CHUNKS_PER_PACKET : nchunks - c0);
+    for (int i = threadIdx.x; i < n; i += 256) sum[i] = chunks[c0 + i] & 3u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned e = 0;
+        for (int i = 0; i < n; i++) {
+            const unsigned v = sum[i];
+            sum[i] = v | (e << 2);
+            e = (v >> e) & 1u;
+        }
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256) chunks[c0 + i] = sum[i];
 }
 
 // Per wave: masks and the switch list of the walk through chunk c; returns the entry parity.
@@ -432,7 +438,7 @@ extern "C" int hrrt_lm_histogram(hrrt_delays *c, const uint32_t *d_words, long l
         const unsigned grid = (unsigned)((nchunks + WAVES - 1) / WAVES);
         const long long npackets = (nchunks + CHUNKS_PER_PACKET - 1) / CHUNKS_PER_PACKET;
         lm_summary_kernel<<<grid, WAVES * 64, 0, s>>>(a, nchunks);
-        lm_resolve_kernel<<<(unsigned)((npackets + 255) / 256), 256, 0, s>>>(a.chunks, nchunks);
+        lm_resolve_kernel<<<(unsigned)npackets, 256, 0, s>>>(a.chunks, nchunks);
         if (opt->start_ms > 0) lm_window_kernel<0><<<grid, WAVES * 64, 0, s>>>(a, nchunks);
         if (opt->end_ms > 0) lm_window_kernel<1><<<grid, WAVES * 64, 0, s>>>(a, nchunks);
         lm_hist_kernel<<<grid, WAVES * 64, 0, s>>>(a, nchunks);
