@@ -106,6 +106,8 @@ SIGNATURES = {
     "hrrt_forward_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_back_proj3d": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_back_proj3d_incremental": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "hrrt_back_proj3d_rotframe": (_I, [_VP, _VP, _VP, _I, _I, _VP]),
+    "hrrt_rotate_add": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_set_backproject_schedule": (_I, [_VP, _I]),
     "hrrt_get_backproject_schedule": (_I, [_VP, _VP]),
     "hrrt_set_psf": (_I, [_VP, _I, C.c_float, C.c_float, C.c_float]),
@@ -122,6 +124,8 @@ SIGNATURES = {
     "hrrt_normfac_calculate": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_forward_proj3d_host": (_I, [_VP, _VP, _VP, _I]),
     "hrrt_back_proj3d_host": (_I, [_VP, _VP, _VP, _I]),
+    "hrrt_back_proj3d_rotframe_host": (_I, [_VP, _VP, _VP, _I, _I]),
+    "hrrt_rotate_add_host": (_I, [_VP, _VP, _VP, _I]),
     # include/hrrt_randoms.h
     "hrrt_delays_create": (_I, [C.POINTER(DelaysParams), C.POINTER(_VP)]),
     "hrrt_delays_destroy": (None, [_VP]),

@@ -47,7 +47,8 @@ struct hrrt_ctx {
     float *pc1 = nullptr;            // [2][N][spp]
     int *colmask = nullptr;          // [views][N]
     uint8_t *gapmask = nullptr;      // [th][def_angles][def_elements]
-    float *h_img = nullptr, *h_prj = nullptr;  // device staging for the host shims
+    float *h_img = nullptr, *h_prj = nullptr, *h_buf = nullptr;  // device staging for the host shims
+    float *rot_zero = nullptr;       // a zero rotated-frame half, for rotating a caller's buffer
     int device = 0;
     int bp_threads = 0;              // back-projection schedule (the reference's -T): 0 = direct for every pair
     // PSF resolution model (-B): host kernels (x, y, z), normz, their device copies, and three
@@ -184,7 +185,7 @@ struct hrrt_ctx {
         drop_graphs();
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
-        void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, (void *)dg.nzmask, f_ratio, fA, fB,
+        void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, h_buf, rot_zero, (void *)dg.nzmask, f_ratio, fA, fB,
                      ll_part, d_psf, psf_img, psf_corr, psf_buf, flat};
         for (void *p : s)
             if (p) (void)hipFree(p);
@@ -1005,6 +1006,31 @@ extern "C" int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *
     return HRRT_OK;
 }
 
+extern "C" int hrrt_back_proj3d_rotframe(hrrt_ctx *ctx, float *d_prj, float *d_imagebuf, int view, int incremental,
+                                         void *stream) {
+    if (!ctx || !d_prj || !d_imagebuf) return fail(HRRT_ERR_ARG, "null argument");
+    if (view < 0 || view >= ctx->geo.views / 2) return fail(HRRT_ERR_ARG, "view %d out of range", view);
+    if (incremental && !ctx->dg.inc_ok)
+        return fail(HRRT_ERR_UNSUPPORTED, "incremental back projection is not available for this geometry");
+    int r;
+    if ((r = ctx->ensure_pair_scratch())) return r;
+    hrrt::PairBatch b{view, 0, 1};
+    HIP_TRY(hrrt::launch_estimate_to_paircol(ctx->dg, d_prj, ctx->pc1, S(stream)));
+    if (incremental) HIP_TRY(hrrt::launch_backproject_incremental(ctx->dg, b, ctx->pc1, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    else HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->pc1, 0, 0, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    HIP_TRY(hrrt::launch_halves_sum(ctx->dg, ctx->bufA, ctx->bufB, d_imagebuf, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_rotate_add(hrrt_ctx *ctx, const float *d_imagebuf, float *d_ima, int view, void *stream) {
+    if (!ctx || !d_imagebuf || !d_ima) return fail(HRRT_ERR_ARG, "null argument");
+    if (view < 0 || view >= ctx->geo.views / 2) return fail(HRRT_ERR_ARG, "view %d out of range", view);
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->rot_zero, sizeof(float) * ctx->vol_floats()))) return r;
+    HIP_TRY(hrrt::launch_rotate_add(ctx->dg, view, d_imagebuf, ctx->rot_zero, d_ima, S(stream)));
+    return HRRT_OK;
+}
+
 extern "C" int hrrt_set_backproject_schedule(hrrt_ctx *ctx, int nthreads) {
     if (!ctx) return fail(HRRT_ERR_ARG, "null context");
     if (nthreads < 0) return fail(HRRT_ERR_ARG, "nthreads must be >= 0");
@@ -1345,6 +1371,31 @@ extern "C" int hrrt_back_proj3d_host(hrrt_ctx *ctx, float *prj, float *ima, int 
     HIP_TRY(hipMemcpy(ctx->h_prj, prj, np, hipMemcpyHostToDevice));
     if ((r = hrrt_back_proj3d(ctx, ctx->h_prj, ctx->h_img, view, nullptr))) return r;
     HIP_TRY(hipMemcpy(prj, ctx->h_prj, np, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ima, ctx->h_img, ni, hipMemcpyDeviceToHost));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_back_proj3d_rotframe_host(hrrt_ctx *ctx, float *prj, float *imagebuf, int view, int incremental) {
+    if (!ctx || !prj || !imagebuf) return fail(HRRT_ERR_ARG, "null argument");
+    const size_t ni = sizeof(float) * ctx->vol_floats(), np = sizeof(float) * 2 * (size_t)ctx->geo.segmentsize * ctx->geo.N;
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->h_img, ni)) || (r = ctx->zalloc((void **)&ctx->h_prj, np))) return r;
+    HIP_TRY(hipMemcpy(ctx->h_img, imagebuf, ni, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->h_prj, prj, np, hipMemcpyHostToDevice));
+    if ((r = hrrt_back_proj3d_rotframe(ctx, ctx->h_prj, ctx->h_img, view, incremental, nullptr))) return r;
+    HIP_TRY(hipMemcpy(prj, ctx->h_prj, np, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(imagebuf, ctx->h_img, ni, hipMemcpyDeviceToHost));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_rotate_add_host(hrrt_ctx *ctx, const float *imagebuf, float *ima, int view) {
+    if (!ctx || !imagebuf || !ima) return fail(HRRT_ERR_ARG, "null argument");
+    const size_t ni = sizeof(float) * ctx->vol_floats();
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->h_img, ni)) || (r = ctx->zalloc((void **)&ctx->h_buf, ni))) return r;
+    HIP_TRY(hipMemcpy(ctx->h_buf, imagebuf, ni, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->h_img, ima, ni, hipMemcpyHostToDevice));
+    if ((r = hrrt_rotate_add(ctx, ctx->h_buf, ctx->h_img, view, nullptr))) return r;
     HIP_TRY(hipMemcpy(ima, ctx->h_img, ni, hipMemcpyDeviceToHost));
     return HRRT_OK;
 }

@@ -92,6 +92,7 @@ hipError_t launch_to_half(const float *in, uint16_t *out, size_t n, hipStream_t 
 hipError_t launch_count_moments(const DevGeom &g, const int16_t *prompt, const float *add, const float *invnorm,
                                 const float *est, int view0, int view_stride, int npairs, double *part,
                                 double *out, hipStream_t s);
+hipError_t launch_halves_sum(const DevGeom &g, const float *bufA, const float *bufB, float *out, hipStream_t s);
 hipError_t launch_rotate_add(const DevGeom &g, int view, const float *bufA, const float *bufB, float *ima,
                              hipStream_t s);
 hipError_t launch_paircol_to_estimate(const DevGeom &g, int view, const float *paircol, float *prj, hipStream_t s);

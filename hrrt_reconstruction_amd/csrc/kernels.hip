@@ -953,6 +953,19 @@ __global__ void __launch_bounds__(256) rot_add_kernel(const DevGeom g, int view,
     ima[i] = add4(ima[i], r);
 }
 
+// The rotated-frame buffer of one view pair, R = A + B on the FOV columns: the reference's
+// imagebuf after back_proj3d_thread1 (back_proj3d_view_n1thread PJ:4162-4668), whose two
+// halves it writes with = and += (the sum is the same single addition).
+__global__ void __launch_bounds__(256) halves_sum_kernel(const DevGeom g, const float4 *__restrict__ A,
+                                                         const float4 *__restrict__ B, float4 *__restrict__ out) {
+    int x, yg;
+    rot_block(blockIdx.x, g.N, (g.N + 3) / 4, x, yg);
+    const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
+    if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
+    const size_t i = ((size_t)x * g.N + y) * g.NQ + q;
+    out[i] = add4(A[i], B[i]);
+}
+
 // ---------------------------------------------------------------------------
 // Layout conversions for the per-view-pair API.
 // ---------------------------------------------------------------------------
@@ -1378,6 +1391,12 @@ hipError_t launch_count_moments(const DevGeom &g, const int16_t *prompt, const f
     hipLaunchKernelGGL(count_moments_kernel, grid, dim3(256), 0, s, g, prompt, add, invnorm, est, view0, view_stride,
                        part);
     hipLaunchKernelGGL(moments_reduce_kernel, dim3(3), dim3(256), 0, s, part, (int)(grid.x * grid.y * grid.z), out);
+    return hipGetLastError();
+}
+
+hipError_t launch_halves_sum(const DevGeom &g, const float *bufA, const float *bufB, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(halves_sum_kernel, rot_grid(g), dim3(64, 4), 0, s, g, reinterpret_cast<const float4 *>(bufA),
+                       reinterpret_cast<const float4 *>(bufB), reinterpret_cast<float4 *>(out));
     return hipGetLastError();
 }
 

@@ -233,6 +233,16 @@ int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *
  * HRRT_ERR_UNSUPPORTED when a breakpoint of the geometry moves the plane range
  * by other than one plane (none does for the HRRT models). */
 int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream);
+/* back_proj3d_thread1 (PJ:1622-1699) as the reference's driver uses it: rescales d_prj in
+ * place and back-projects both views of the pair into d_imagebuf [X][Y][Z4], in the rotated
+ * frame of `view`, without rotating.  The FOV columns are overwritten; the rest of d_imagebuf
+ * is left untouched.  incremental = 1 takes the incremental projector (back_proj3d_thread1),
+ * 0 the direct one (the rotated-frame half of back_proj3d2). */
+int hrrt_back_proj3d_rotframe(hrrt_ctx *ctx, float *d_prj, float *d_imagebuf, int view, int incremental,
+                              void *stream);
+/* One buffer of rotate_image_buf (PJ:1701-1725, pt_rotate3d2): d_ima += rotate(d_imagebuf,
+ * view), the bilinear rotation of the back projectors, over the FOV columns of d_ima. */
+int hrrt_rotate_add(hrrt_ctx *ctx, const float *d_imagebuf, float *d_ima, int view, void *stream);
 /* Back-projection schedule of the FP32 subset updates (hrrt_osem_subset, *_frames) and of
  * hrrt_calculate_normfac, the reference's -T nthreads (S3:3796-3825, 4174-4218): the first
  * npairs - npairs % nthreads view pairs of each subset take the incremental projector
@@ -294,6 +304,8 @@ int hrrt_normfac_calculate(hrrt_ctx *ctx, const float *d_sens, float *d_out, voi
  * written against the reference's functions binds to. */
 int hrrt_forward_proj3d_host(hrrt_ctx *ctx, const float *image, float *prj, int view);
 int hrrt_back_proj3d_host(hrrt_ctx *ctx, float *prj, float *ima, int view);
+int hrrt_back_proj3d_rotframe_host(hrrt_ctx *ctx, float *prj, float *imagebuf, int view, int incremental);
+int hrrt_rotate_add_host(hrrt_ctx *ctx, const float *imagebuf, float *ima, int view);
 
 #ifdef __cplusplus
 }
