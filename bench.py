@@ -454,7 +454,7 @@ def main():
         randoms = randoms_leg(torch, args.randoms_steps, args.seed + 11, [1, 3] if args.cpu_subsets > 0 else None,
                               args.randoms_span)
     listmode = None
-    if args.listmode_events > 0:
+    if args.listmode_events > 0 and args.listmode_steps > 0:
         listmode = listmode_leg(torch, args.listmode_events, args.listmode_steps, args.seed + 13,
                                 20_000_000 if args.cpu_subsets > 0 else 0, GEOM["span"])
     batched = None

@@ -146,6 +146,88 @@ __device__ __forceinline__ void sample8_store(const DevGeom &g, const SampleRegs
     }
 }
 
+// Transposed staging of a row (the FAST forward path).  A wave owns 16 chunks (4 planes
+// each) of one half of the row's samples: lane l samples chunk 16 b + (l & 15) of sample
+// 4 h + (l >> 4), so each row of 16 lanes loads 256 contiguous bytes of one image column.
+// A 4x4 transpose across the four 16-lane rows (two permlane32 swaps, then two permlane16
+// swaps) leaves lane l with plane 4 q + (l >> 4) of all four samples, stored with one
+// 16-byte LDS write instead of four scattered 4-byte ones.
+__device__ __forceinline__ int xpose_chunks(const DevGeom &g) { return 2 * ((g.NQ + 15) >> 4); }
+
+__device__ __forceinline__ void xpose_item(const DevGeom &g, int it, int &h, int &q, int &i) {
+    const int nq16 = (g.NQ + 15) >> 4, chunk = it >> 6, l = it & 63;
+    h = chunk >= nq16;
+    q = 16 * (chunk - h * nq16) + (l & 15);
+    i = 4 * h + (l >> 4);
+}
+
+__device__ __forceinline__ void swap_pl32(float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ void swap_pl16(float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+// v: planes 4q..4q+3 of sample (l >> 4) -> plane 4q + (l >> 4) of samples 0..3.
+__device__ __forceinline__ float4 xpose4(float4 v) {
+    swap_pl32(v.x, v.z);   // rows 0-1 receive plane 0 of samples 2-3, rows 2-3 plane 2 of samples 0-1
+    swap_pl32(v.y, v.w);
+    swap_pl16(v.x, v.y);   // odd rows' x <-> even rows' y
+    swap_pl16(v.z, v.w);
+    return v;
+}
+
+template <int ITEMS>
+__device__ __forceinline__ void sample8x_load(const DevGeom &g, const float4 *__restrict__ image, float cosv,
+                                              float sinv, int x, int y, SampleRegs<ITEMS> &s) {
+    const int NQ = g.NQ, N = g.N, c = g.c;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int it = threadIdx.x + j * blockDim.x;
+        s.ok[j] = false;
+        if ((it >> 6) >= xpose_chunks(g)) continue;
+        int h, q, i, xx, yy;
+        xpose_item(g, it, h, q, i);
+        if (q >= NQ) continue;
+        sym8(i, x, y, N, xx, yy);
+        const float xx0 = (xx - c) * cosv - (yy - c) * sinv + c;
+        const float yy0 = (xx - c) * sinv + (yy - c) * cosv + c;
+        if (xx0 < 0 || xx0 > N - 2 || yy0 < 0 || xx0 > N - 2) continue;
+        const int xi = (int)xx0, yi = (int)yy0;
+        const float x1 = xx0 - xi, y1 = yy0 - yi;
+        s.w1[j] = (1 - x1) * (1 - y1); s.w2[j] = x1 * (1 - y1); s.w3[j] = y1 * (1 - x1); s.w4[j] = y1 * x1;
+        const float4 *p = image + ((size_t)xi * N + yi) * NQ + q;
+        s.r1[j] = p[0];
+        s.r2[j] = p[(size_t)N * NQ];
+        s.r3[j] = p[NQ];
+        s.r4[j] = p[(size_t)N * NQ + NQ];
+        s.ok[j] = true;
+    }
+}
+
+template <int ITEMS>
+__device__ __forceinline__ void sample8x_store(const DevGeom &g, const SampleRegs<ITEMS> &s, float4 *pam,
+                                               float4 *pam2) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int it = threadIdx.x + j * blockDim.x;
+        if ((it >> 6) >= xpose_chunks(g)) continue;   // whole waves
+        int h, q, i;
+        xpose_item(g, it, h, q, i);
+        float4 v = s.ok[j] ? add4(add4(mul4(s.r1[j], s.w1[j]), mul4(s.r2[j], s.w2[j])),
+                                  add4(mul4(s.r3[j], s.w3[j]), mul4(s.r4[j], s.w4[j])))
+                           : f4(0.f);
+        v = xpose4(v);
+        const int z = 4 * q + (i & 3);
+        if (q < g.NQ && z < g.Z) (h ? pam2 : pam)[z] = v;
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (blockIdx k runs on XCD k mod 8),
 // each with its own L2.  This maps a dispatch index to a logical index such that every XCD
 // receives one contiguous range of logical indices, processed in order.
@@ -295,12 +377,8 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
         acc2[t] = v;
     }
 
-    // one row: project the staged row y (buffer ST) while the next row's samples load
-    auto row = [&](auto stc, int y) {
-        constexpr int st = decltype(stc)::value;
-        SampleRegs<ITEMS> nxt;
-        if (y + 1 < c) sample8_load<ITEMS>(g, image, cosv, sinv, x, y + 1, nxt);
-        const float4 *pa = stage + st * 2 * PL, *pb = pa + PL;
+    // project the staged row y (halves pa, pb) into every plane this thread owns
+    auto project = [&](const float4 *pa, const float4 *pb, int y) {
         const float fy2 = (float)(c - y);   // (y - c) * tan = -((c - y) * tan) exactly
 #pragma unroll
         for (int t = 0; t < MAXT; t++) {
@@ -331,7 +409,22 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
             acc1[t] = add4(acc1[t], add4(mul4(add4(d1, a0), f2), mul4(add4(a1, d0), f1)));
             acc2[t] = add4(acc2[t], add4(mul4(add4(e1, b0), f2), mul4(add4(b1, e0), f1)));
         }
-        if (y + 1 < c) sample8_store<ITEMS>(g, nxt, stage + (st ^ 1) * 2 * PL, stage + (st ^ 1) * 2 * PL + PL);
+    };
+    // one row per barrier: project row y (buffer ST) while the next row's samples load
+    auto row = [&](auto stc, int y) {
+        constexpr int st = decltype(stc)::value;
+        SampleRegs<ITEMS> nxt;
+        if (y + 1 < c) {
+            if (FAST) sample8x_load<ITEMS>(g, image, cosv, sinv, x, y + 1, nxt);
+            else sample8_load<ITEMS>(g, image, cosv, sinv, x, y + 1, nxt);
+        }
+        const float4 *pa = stage + st * 2 * PL;
+        project(pa, pa + PL, y);
+        if (y + 1 < c) {
+            float4 *na = stage + (st ^ 1) * 2 * PL;
+            if (FAST) sample8x_store<ITEMS>(g, nxt, na, na + PL);
+            else sample8_store<ITEMS>(g, nxt, na, na + PL);
+        }
         __syncthreads();
     };
     for (int y = ystart; y < c; y += 2) {   // row ystart + 2j reads buffer 0, the next one buffer 1
@@ -1314,7 +1407,8 @@ hipError_t launch_forward(const DevGeom &g, const PairBatch &b, const float *ima
     case M:                                                                                           \
         return it2 <= 1 ? fp_launch_t<M, 1, false>(g, b, image, out, io, mode, nsplit, threads, s)    \
                         : fp_launch_t<M, 2, false>(g, b, image, out, io, mode, nsplit, threads, s);
-    if (maxt == 4 && it2 <= 1 && g.fp_fast && g.Z4 + 8 == FP_PL_FAST)
+    // the FAST path stages a row as 2 x ceil(NQ/16) whole waves of samples, one item each
+    if (maxt == 4 && g.fp_fast && g.Z4 + 8 == FP_PL_FAST && 128 * ((g.NQ + 15) / 16) <= threads)
         return fp_launch_t<4, 1, true>(g, b, image, out, io, mode, nsplit, threads, s);
     switch (maxt) {
         FP_CASE(1)
