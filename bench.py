@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 _T0 = time.perf_counter()
 
 
@@ -50,6 +51,23 @@ def algorithmic_bytes(g, npairs):
         "backproject": bins * 4 + 2 * npairs * vol,
         # both halves of every pair read; image and 1/sensitivity read, image written
         "rotate_update": 2 * npairs * vol + 3 * vol,
+    }
+
+
+def algorithmic_flops(g, npairs):
+    """FP32 operations each kernel must do per launch (one subset), independent of the
+    implementation: an axial interpolation and accumulation (4 operations) per sinogram plane
+    and FOV pixel of every view, a bilinear rotation (7) per FOV voxel of every view, the ratio
+    (3) per bin and the update (4) per voxel."""
+    d = g.dims
+    views = 2 * npairs
+    lerp = views * g.S * d.fov_voxels * 4
+    rotate = views * d.fov_voxels * g.Z * 7
+    bins = npairs * 2 * g.S * d.fov_bins
+    return {
+        "fp_ratio": lerp + rotate + 3 * bins,      # the image rotated into each view, then the rays
+        "backproject": lerp,
+        "rotate_update": rotate + 4 * d.fov_voxels * g.Z,
     }
 
 
@@ -438,6 +456,9 @@ def main():
     dom_ms = kt[dom][0] / max(kt[dom][1], 1)
     achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
     traffic = read_traffic(dom)
+    flops = algorithmic_flops(g, npairs)
+    dom_tflops = flops[dom] / (dom_ms * 1e-3) / 1e12
+    subset_tflops = sum(flops.values()) * value / 1e12
 
     cpu = None
     if args.cpu_subsets > 0:
@@ -483,8 +504,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic},
+        "compute_roofline": {"bound": "fp32", "kernel": dom, "achieved": round(dom_tflops, 3),
+                             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(dom_tflops / FP32_PEAK_TFLOPS, 5),
+                             "subset_achieved": round(subset_tflops, 3),
+                             "subset_frac": round(subset_tflops / FP32_PEAK_TFLOPS, 5)},
         "kernels": {k: {"ms_per_launch": round(v[0] / max(v[1], 1), 4), "launches": v[1],
-                        "alg_bytes": alg[k]} for k, v in kt.items()},
+                        "alg_bytes": alg[k], "alg_flops": flops[k]} for k, v in kt.items()},
         "cpu_baseline": cpu,
         "psf": psf,
         "randoms": randoms,
