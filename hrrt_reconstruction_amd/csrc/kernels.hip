@@ -1048,7 +1048,8 @@ __global__ void __launch_bounds__(256) rot_add_kernel(const DevGeom g, int view,
 
 // The rotated-frame buffer of one view pair, R = A + B on the FOV columns: the reference's
 // imagebuf after back_proj3d_thread1 (back_proj3d_view_n1thread PJ:4162-4668), whose two
-// halves it writes with = and += (the sum is the same single addition).
+// halves it writes with = and += (the sum is the same single addition).  Like the reference,
+// it leaves the pad planes z >= z_pixels as they were.
 __global__ void __launch_bounds__(256) halves_sum_kernel(const DevGeom g, const float4 *__restrict__ A,
                                                          const float4 *__restrict__ B, float4 *__restrict__ out) {
     int x, yg;
@@ -1056,7 +1057,14 @@ __global__ void __launch_bounds__(256) halves_sum_kernel(const DevGeom g, const 
     const int q = threadIdx.x, y = yg * blockDim.y + threadIdx.y;
     if (q >= g.NQ || y < g.cyl_lo[x] || y >= g.cyl_hi[x]) return;
     const size_t i = ((size_t)x * g.N + y) * g.NQ + q;
-    out[i] = add4(A[i], B[i]);
+    float4 r = add4(A[i], B[i]);
+    if (4 * q + 3 >= g.Z) {
+        const float4 o = out[i];
+        const int z = 4 * q;
+        r = make_float4(z + 0 < g.Z ? r.x : o.x, z + 1 < g.Z ? r.y : o.y, z + 2 < g.Z ? r.z : o.z,
+                        z + 3 < g.Z ? r.w : o.w);
+    }
+    out[i] = r;
 }
 
 // ---------------------------------------------------------------------------

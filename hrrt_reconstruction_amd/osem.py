@@ -317,6 +317,19 @@ class Osem3D(Layout):
         _lib.call("hrrt_back_proj3d", self.ctx, _p(prj), _p(ima), int(view), _stream())
         return ima
 
+    def back_proj3d_rotframe(self, prj, imagebuf, view: int, incremental: bool = True):
+        """back_proj3d_thread1 as the reference driver uses it (PJ:1622-1699): rescales prj in
+        place and back-projects the pair into the FOV columns of imagebuf, in the rotated frame
+        of `view`, without rotating (incremental=False: the direct projector)."""
+        _lib.call("hrrt_back_proj3d_rotframe", self.ctx, _p(prj), _p(imagebuf), int(view), int(bool(incremental)),
+                  _stream())
+        return imagebuf
+
+    def rotate_add(self, imagebuf, ima, view: int):
+        """One buffer of rotate_image_buf (PJ:1701-1725): ima += rotate(imagebuf, view)."""
+        _lib.call("hrrt_rotate_add", self.ctx, _p(imagebuf), _p(ima), int(view), _stream())
+        return ima
+
     def update_estimate_w3(self, est, add, prompt):
         _lib.call("hrrt_update_estimate_w3", self.ctx, _p(est), _p(add), _p(prompt), _stream())
         return est

@@ -45,8 +45,8 @@ int forward_proj3d_thread1(float ***ima, float **prj, int view, int numthread, f
  * the rotated result into ima. */
 int back_proj3d2(float **prj, float ***ima, int view, int numthread, float ***imagebuf, float ***prjbuf);
 /* back_proj3d_thread1 (PJ:1622): rescales prj in place and back-projects (incremental
- * projector) into the FOV columns of imagebuf, in the rotated frame of `view`; ima is not
- * used.  The reference's driver follows it with rotate_image_buf. */
+ * projector) into the FOV columns of imagebuf (planes z < z_pixels), in the rotated frame of
+ * `view`; ima is not used.  The reference's driver follows it with rotate_image_buf. */
 int back_proj3d_thread1(float **prj, float ***ima, int view, int numthread, float ***imagebuf, float ***prjbuf);
 /* rotate_image_buf (PJ:1701-1725): for i in 0..numthread-1, ima += rotate(imagebuf[i], view[i]). */
 int rotate_image_buf(float ****imagebuf, float ***ima, int numthread, int *view);

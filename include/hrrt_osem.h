@@ -235,8 +235,10 @@ int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *
 int hrrt_back_proj3d_incremental(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream);
 /* back_proj3d_thread1 (PJ:1622-1699) as the reference's driver uses it: rescales d_prj in
  * place and back-projects both views of the pair into d_imagebuf [X][Y][Z4], in the rotated
- * frame of `view`, without rotating.  The FOV columns are overwritten; the rest of d_imagebuf
- * is left untouched.  incremental = 1 takes the incremental projector (back_proj3d_thread1),
+ * frame of `view`, without rotating.  The planes z < z_pixels of the FOV columns are
+ * overwritten; the rest of d_imagebuf is left untouched, as the reference leaves it (it also
+ * zeroes the pad planes of the columns it skips for a zero norm; its driver keeps every pad
+ * plane zero, so the two agree there).  incremental = 1 takes the incremental projector (back_proj3d_thread1),
  * 0 the direct one (the rotated-frame half of back_proj3d2). */
 int hrrt_back_proj3d_rotframe(hrrt_ctx *ctx, float *d_prj, float *d_imagebuf, int view, int incremental,
                               void *stream);

@@ -101,6 +101,7 @@ def test_back_proj3d_thread1_and_rotate_image_buf(legacy, oracle_lib, geom):
         assert legacy.back_proj3d_thread1(prj.ptr, None, view, 4, ib.ptr, None) == 1
         assert np.array_equal(prj.a[:, : o.N], ref_prj), "in-place rescale of the estimate differs"
         expect = np.where(fov[:, :, None], ref_ib, start)
+        expect[:, :, o.Z:] = start[:, :, o.Z:]         # pad planes: left as they were
         assert np.array_equal(ib.a[:, :, : o.Z4], expect), f"view {view}"
         bufs.append(ib)
         refs.append(expect)
@@ -145,6 +146,44 @@ def test_norm_zero_mask(legacy, oracle_lib):
         ib = Image3(padded_image(o.image_zeros()))
         assert legacy.back_proj3d_thread1(Rows2(padded_rows(prj0)).ptr, None, view, 1, ib.ptr, None) == 1
         assert np.array_equal(ib.a[:, :, : o.Z4], np.where(fov[:, :, None], ref_ib, 0.0)), f"back view {view}"
+
+
+@pytest.mark.parametrize("geom,view", [("tiny", 5), ("c1", 33)])
+def test_rotframe_and_rotate_add_entry_points(legacy, oracle_lib, geom, view):
+    """The C ABI forms under the legacy symbols: hrrt_back_proj3d_rotframe (both projectors)
+    and hrrt_rotate_add on device buffers, and their _host variants on host buffers."""
+    import torch
+    from gpu_util import from_dev, to_dev
+    from hrrt_reconstruction_amd import _lib
+    from hrrt_reconstruction_amd.osem import Osem3D
+    print(f"rotframe / rotate_add geom={geom} view={view}", flush=True)
+    o = oracle_lib.Oracle(**GEOMS[geom])
+    g = Osem3D("cuda:0", **GEOMS[geom])
+    fov = _fov(o)
+    rng = np.random.default_rng(71 + view)
+    prj0 = rng.uniform(0.0, 2.0, size=(2 * o.S, o.N)).astype(np.float32)
+    stale = rng.uniform(-1.0, 1.0, (o.N, o.N, o.Z4)).astype(np.float32)
+    for incremental, og in ((True, o.L.og_back_proj_incremental), (False, o.L.og_back_proj_direct_rotframe)):
+        ref_prj, ref_ib = prj0.copy(), o.image_zeros()
+        og(o.gp, _vp(ref_prj), _vp(ref_ib), view, _vp(o.prjbuf()))
+        expect = np.where(fov[:, :, None], ref_ib, stale)
+        expect[:, :, o.Z:] = stale[:, :, o.Z:]
+        d_prj, d_ib = to_dev(prj0), to_dev(stale)
+        g.back_proj3d_rotframe(d_prj, d_ib, view, incremental)
+        assert np.array_equal(from_dev(d_prj), ref_prj) and np.array_equal(from_dev(d_ib), expect), incremental
+        h_prj, h_ib = prj0.copy(), stale.copy()
+        _lib.call("hrrt_back_proj3d_rotframe_host", g.ctx, _vp(h_prj), _vp(h_ib), view, int(incremental))
+        assert np.array_equal(h_prj, ref_prj) and np.array_equal(h_ib, expect), incremental
+    ib = np.where(fov[:, :, None], o.back_incremental_rotframe(prj0, view), 0.0).astype(np.float32)
+    ima0 = random_image(o, seed=view)
+    ref = o.rotate_add(ib, view, out=ima0.copy())
+    d_ima = to_dev(ima0)
+    g.rotate_add(to_dev(ib), d_ima, view)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(d_ima), ref)
+    h_ima = ima0.copy()
+    _lib.call("hrrt_rotate_add_host", g.ctx, _vp(ib), _vp(h_ima), view)
+    assert np.array_equal(h_ima, ref)
 
 
 def test_argument_checks(legacy, oracle_lib):
