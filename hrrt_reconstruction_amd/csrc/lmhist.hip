@@ -355,14 +355,38 @@ __global__ void lm_finalize_kernel(const int *__restrict__ cnt, long long S, int
     }
 }
 
+// Eight bins per thread: two 16-byte count loads, one 16-byte sinogram store (and one
+// 8-byte delayed store).  Needs S % 8 == 0 and 16-byte aligned outputs (8 for the bytes).
+__device__ __forceinline__ unsigned pack16(int a, int b) { return ((unsigned)a & 0xffffu) | ((unsigned)b << 16); }
+
+__device__ __forceinline__ unsigned pack8(int a, int b, int c, int d) {
+    return (unsigned)min(a, 255) | ((unsigned)min(b, 255) << 8) | ((unsigned)min(c, 255) << 16) |
+           ((unsigned)min(d, 255) << 24);
+}
+
+__global__ void lm_finalize8_kernel(const int4 *__restrict__ cnt, long long S8, int mode, uint4 *__restrict__ sino,
+                                    uint2 *__restrict__ delayed) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S8) return;
+    const int4 a = cnt[2 * i], b = cnt[2 * i + 1];
+    sino[i] = make_uint4(pack16(a.x, a.y), pack16(a.z, a.w), pack16(b.x, b.y), pack16(b.z, b.w));
+    if (mode == 1) {
+        const int4 c = cnt[2 * (S8 + i)], d = cnt[2 * (S8 + i) + 1];
+        if (delayed) delayed[i] = make_uint2(pack8(c.x, c.y, c.z, c.w), pack8(d.x, d.y, d.z, d.w));
+        else sino[S8 + i] = make_uint4(pack16(c.x, c.y), pack16(c.z, c.w), pack16(d.x, d.y), pack16(d.z, d.w));
+    }
+}
+
 __global__ void lm_stats_init_kernel(unsigned long long *st) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ST_WORDS) st[i] = (i == ST_START || i == ST_END) ? NONE : 0ull;
 }
 
 __global__ void lm_zero_kernel(unsigned *p, long long n) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        p[i] = 0u;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+    const long long n4 = (uintptr_t)p % 16 == 0 ? n / 4 : 0;     // 16-byte stores where aligned
+    for (long long i = t; i < n4; i += stride) reinterpret_cast<uint4 *>(p)[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (long long i = 4 * n4 + t; i < n; i += stride) p[i] = 0u;
 }
 
 // The module-pair LUT by (module pair, A crystal, B crystal) on the device, and the counts.
@@ -443,7 +467,11 @@ extern "C" int hrrt_lm_histogram(hrrt_delays *c, const uint32_t *d_words, long l
         if (opt->end_ms > 0) lm_window_kernel<1><<<grid, WAVES * 64, 0, s>>>(a, nchunks);
         lm_hist_kernel<<<grid, WAVES * 64, 0, s>>>(a, nchunks);
     }
-    lm_finalize_kernel<<<(unsigned)((S + 255) / 256), 256, 0, s>>>(a.cnt, S, opt->mode, d_sino, d_delayed);
+    if (S % 8 == 0 && (uintptr_t)d_sino % 16 == 0 && (uintptr_t)d_delayed % 8 == 0)
+        lm_finalize8_kernel<<<(unsigned)((S / 8 + 255) / 256), 256, 0, s>>>(
+            (const int4 *)a.cnt, S / 8, opt->mode, (uint4 *)d_sino, (uint2 *)d_delayed);
+    else
+        lm_finalize_kernel<<<(unsigned)((S + 255) / 256), 256, 0, s>>>(a.cnt, S, opt->mode, d_sino, d_delayed);
     DTRY(hipGetLastError());
     if (stats) {
         std::vector<unsigned long long> h(ST_WORDS);
