@@ -383,10 +383,8 @@ __global__ void lm_stats_init_kernel(unsigned long long *st) {
 }
 
 __global__ void lm_zero_kernel(unsigned *p, long long n) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
-    const long long n4 = (uintptr_t)p % 16 == 0 ? n / 4 : 0;     // 16-byte stores where aligned
-    for (long long i = t; i < n4; i += stride) reinterpret_cast<uint4 *>(p)[i] = make_uint4(0u, 0u, 0u, 0u);
-    for (long long i = 4 * n4 + t; i < n; i += stride) p[i] = 0u;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 // The module-pair LUT by (module pair, A crystal, B crystal) on the device, and the counts.
