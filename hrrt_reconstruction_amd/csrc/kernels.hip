@@ -360,17 +360,18 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
     __syncthreads();
 
     float4 acc1[MAXT], acc2[MAXT];
-    int code[MAXT], ymin[MAXT];
+    int code[MAXT], ymin[MAXT], dbyte[MAXT];
     float pbase[MAXT], ptan[MAXT];
 #pragma unroll
     for (int t = 0; t < MAXT; t++) {
         const int p = p_lo + threadIdx.x + t * blockDim.x;
-        code[t] = -1; ymin[t] = c; pbase[t] = 0.f; ptan[t] = 0.f;
+        code[t] = -1; ymin[t] = c; pbase[t] = 0.f; ptan[t] = 0.f; dbyte[t] = 0;
         int yr = 0;
         if (p < p_hi) {
             const int4 e = g.fp_planes[p];
             code[t] = e.x; pbase[t] = __int_as_float(e.y); ptan[t] = __int_as_float(e.z); ymin[t] = e.w;
             yr = (e.x >> 8) & 0xff;
+            dbyte[t] = (e.x >> 16) * (int)sizeof(float4);   // the plane shift delta, in LDS bytes
         }
         const float4 v = rimc[yr];   // centre row sample at plane yr (PJ:1830-1838)
         acc1[t] = v;
@@ -390,7 +391,6 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
                 continue;
             }
             if (y < ymin[t]) continue;   // PJ:1920-1921
-            const int delta = code[t] >> 16;
             const float pt = fy2 * ptan[t];
             const float zz1 = pbase[t] - pt, zz2 = pbase[t] + pt;
             const int i1 = (int)zz1, i2 = (int)zz2;
@@ -402,9 +402,11 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
                 f1 = zz1 - i1;
                 f2 = zz2 - i2;
             }
-            const int z = i1 + delta, z2 = i2 + delta;
-            const float4 a0 = pa[z], a1 = pa[z + 1], b0 = pb[z], b1 = pb[z + 1];
-            const float4 d0 = pb[z2], d1 = pb[z2 + 1], e0 = pa[z2], e1 = pa[z2 + 1];
+            // planes z = i1 + delta and z2 = i2 + delta: the shift folded into the base address
+            const float4 *qa = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(pa) + dbyte[t]);
+            const float4 *qb = reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(pb) + dbyte[t]);
+            const float4 a0 = qa[i1], a1 = qa[i1 + 1], b0 = qb[i1], b1 = qb[i1 + 1];
+            const float4 d0 = qb[i2], d1 = qb[i2 + 1], e0 = qa[i2], e1 = qa[i2 + 1];
             // PJ:1934-1944: (pam2[z2+1] + pam[z])*f2 + (pam[z+1] + pam2[z2])*f1 and its mirror
             acc1[t] = add4(acc1[t], add4(mul4(add4(d1, a0), f2), mul4(add4(a1, d0), f1)));
             acc2[t] = add4(acc2[t], add4(mul4(add4(e1, b0), f2), mul4(add4(b1, e0), f1)));
