@@ -68,17 +68,15 @@ __device__ __forceinline__ void put_lanes(float4 *buf, int q, int lanei, float4 
 
 // The eight symmetric sample positions of (x, y) (PJ:1856-1864).  Samples 0..3
 // feed "pam" (near half of the ray), 4..7 feed "pam2" (mirrored half).
+// i: (x, y), (N-x, y), (N-y, x), (N-y, N-x), (x, N-y), (N-x, N-y), (y, x), (y, N-x).  Bit 0
+// mirrors x, bit 1 swaps the two coordinates, bit 2 mirrors y; as selects rather than a branch
+// tree, since the sample index differs between the 16-lane rows of a wave.
 __device__ __forceinline__ void sym8(int i, int x, int y, int N, int &xx, int &yy) {
-    switch (i) {
-        case 0: xx = x; yy = y; break;
-        case 1: xx = N - x; yy = y; break;
-        case 2: xx = N - y; yy = x; break;
-        case 3: xx = N - y; yy = N - x; break;
-        case 4: xx = x; yy = N - y; break;
-        case 5: xx = N - x; yy = N - y; break;
-        case 6: xx = y; yy = x; break;
-        default: xx = y; yy = N - x; break;
-    }
+    const int X = (i & 1) ? N - x : x;
+    const int Yd = (i & 4) ? N - y : y;   // y coordinate of the unswapped samples
+    const int Ys = (i & 4) ? y : N - y;   // x coordinate of the swapped samples
+    xx = (i & 2) ? Ys : X;
+    yy = (i & 2) ? X : Yd;
 }
 
 __device__ __forceinline__ void sample8(const DevGeom &g, const float4 *__restrict__ image, float cosv, float sinv,
@@ -201,11 +199,13 @@ __device__ __forceinline__ void sample8x_load(const DevGeom &g, const float4 *__
         const int xi = (int)xx0, yi = (int)yy0;
         const float x1 = xx0 - xi, y1 = yy0 - yi;
         s.w1[j] = (1 - x1) * (1 - y1); s.w2[j] = x1 * (1 - y1); s.w3[j] = y1 * (1 - x1); s.w4[j] = y1 * x1;
-        const float4 *p = image + ((size_t)xi * N + yi) * NQ + q;
-        s.r1[j] = p[0];
-        s.r2[j] = p[(size_t)N * NQ];
-        s.r3[j] = p[NQ];
-        s.r4[j] = p[(size_t)N * NQ + NQ];
+        // 32-bit chunk offsets from the uniform image base (an image holds < 2^32 chunks)
+        const unsigned o = ((unsigned)xi * (unsigned)N + (unsigned)yi) * (unsigned)NQ + (unsigned)q;
+        const unsigned dx = (unsigned)N * (unsigned)NQ;
+        s.r1[j] = image[o];
+        s.r2[j] = image[o + dx];
+        s.r3[j] = image[o + (unsigned)NQ];
+        s.r4[j] = image[o + dx + (unsigned)NQ];
         s.ok[j] = true;
     }
 }
