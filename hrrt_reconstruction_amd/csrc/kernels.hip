@@ -768,7 +768,9 @@ __global__ void __launch_bounds__(256) bp_inc_kernel(const DevGeom g, const Pair
             const int4 nxt = T[e + 1];
             const float cf = __int_as_float(cur.x);
             const int lo8 = cur.w & 0xffff, hi8 = cur.w >> 16;
-            const int ra = clampi(z8 + cur.y, lo8, hi8), rb = clampi(z8 + cur.z, lo8, hi8);
+            // lo8 < hi8 always, so the clamp is max(lo, min(v, hi)): two VALU instead of a compare,
+            // a select and a min
+            const int ra = max(lo8, min(z8 + cur.y, hi8)), rb = max(lo8, min(z8 + cur.z, hi8));
             const float2 ma = *reinterpret_cast<const float2 *>(Wb + ra);
             const float2 mb = *reinterpret_cast<const float2 *>(Wb + rb);
             // b-term (PJ:4386-4391 for rows N-y): s1 += m2*coef, s2 += m1*coef, d1 -= m2, d2 -= m1
