@@ -148,3 +148,22 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=300, cwd=ROOT)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_bench_algorithmic_counts():
+    """The bench's algorithmic work per subset against the figures worked out by hand for the
+    two HRRT workloads: 1.92 GFLOP (span 9, 128², 8 of 9 views used) and 45.1 GFLOP (span 3,
+    256², 18 views), 46441 FOV pixels per plane at 256²; the forward kernel's bytes are the
+    image's FOV columns plus 10 bytes per FOV bin of the subset."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from hrrt_reconstruction_amd.osem import Layout, make_params
+    c1 = Layout(make_params(**GEOMS["c1"]))
+    c2 = Layout(make_params(**GEOMS["c2"]))
+    assert c2.dims.fov_voxels == 46441 and c2.S == 6367 and c2.sviews == 18
+    f1 = bench.algorithmic_flops(c1, c1.sviews // 2)
+    f2 = bench.algorithmic_flops(c2, c2.sviews // 2)
+    assert abs(sum(f1.values()) - 1.9224e9) < 1e6
+    assert abs(sum(f2.values()) - 45.1239e9) < 1e6
+    b2 = bench.algorithmic_bytes(c2, 9)
+    assert b2["fp_ratio"] == 46441 * 208 * 4 + 9 * 2 * 6367 * c2.dims.fov_bins * 10
