@@ -142,3 +142,31 @@ def test_list_mode_to_osem_subset(torch_dev, oracle_lib):
     o.osem_subset(img_ref, o.image_zeros(), np.ascontiguousarray(nf_host[s]), ref_add, ref_prompt, s,
                   nthreads=G.backproject_schedule, workers=8)
     assert np.array_equal(from_dev(image), img_ref)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_unaligned_outputs_take_the_scalar_finalize(torch_dev, stream, mode):
+    """Outputs that are not 16-byte (sinogram) or 8-byte (delayed bytes) aligned go through the
+    one-bin-per-thread finalize; the bins must equal the aligned call's, which the oracle pins."""
+    import ctypes as C
+    from hrrt_reconstruction_amd import _lib, lmhist
+    from hrrt_reconstruction_amd.randoms import GenDelays
+    print(f"lmhist unaligned outputs mode={mode}", flush=True)
+    g = GenDelays(span=9)
+    d_words = torch_dev.from_numpy(stream.view(np.int32)).cuda()
+    ref, ref_d, _, _, _ = lmhist.histogram(g, d_words, mode=mode, start_ms=20100, end_ms=20400)
+    n = g.nplanes * g.nviews * g.nprojs
+    buf = torch_dev.full((n + 8,), 12345, dtype=torch_dev.int16, device="cuda")
+    sino = buf[1:1 + n]                                   # 2 bytes past a 16-byte boundary
+    dbuf = torch_dev.full((n + 8,), 7, dtype=torch_dev.uint8, device="cuda")
+    delayed = dbuf[3:3 + n] if mode == 1 else None       # 3 bytes past an 8-byte boundary
+    opt = _lib.LmOptions(mode, 20100, 20400, 0, 0)
+    p = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+    _lib.call("hrrt_lm_histogram", g._h, p(d_words), d_words.numel(), C.byref(opt), p(sino), p(delayed),
+              None, None, None, C.c_void_p(torch_dev.cuda.current_stream().cuda_stream))
+    torch_dev.cuda.synchronize()
+    assert torch_dev.equal(sino, ref.reshape(-1))
+    assert int(buf[0]) == 12345 and torch_dev.all(buf[1 + n:] == 12345)
+    if mode == 1:
+        assert torch_dev.equal(delayed, ref_d.reshape(-1))
+        assert torch_dev.all(dbuf[:3] == 7) and torch_dev.all(dbuf[3 + n:] == 7)
