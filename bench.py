@@ -508,7 +508,9 @@ def main():
                                "randoms 10%, scatter 15%, 16 subsets (9 view pairs per subset)",
                    "subsets": g.subsets, "view_pairs_per_subset": npairs,
                    "stage_s": round(stage_s, 3), "normfac_s": round(normfac_s, 3),
-                   "fp_launch_shape": list(shape) if shape else None, "back_projector": bp_choice,
+                   "fp_launch_shape": list(shape) if shape else None, "direct_bp_waves": bp_choice,
+                   "back_projector": "incremental" if g.backproject_schedule == 1 else
+                   ("direct" if g.backproject_schedule == 0 else "mixed"),
                    "backproject_schedule": g.backproject_schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),

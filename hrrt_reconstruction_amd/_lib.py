@@ -90,6 +90,8 @@ SIGNATURES = {
     "hrrt_calculate_normfac": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_init_image": (_I, [_VP, _VP, _VP]),
     "hrrt_osem_subset": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "hrrt_stage_sinograms_w012": (_I, [_VP, _I, _I] + [_VP] * 8 + [_VP]),
+    "hrrt_osem_subset_w012": (_I, [_VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_end_iteration": (_I, [_VP, _VP, _VP]),
     "hrrt_osem_subset_frames": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_osem_iteration_frames": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
@@ -120,6 +122,7 @@ SIGNATURES = {
     "hrrt_get_psf_kernel": (_I, [_VP, _I, _VP, _VP]),
     "hrrt_convolve3d": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_update_estimate_w3": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "hrrt_update_estimate_w012": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_cal_updateimage": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "hrrt_normfac_calculate": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_forward_proj3d_host": (_I, [_VP, _VP, _VP, _I]),

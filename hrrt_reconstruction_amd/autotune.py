@@ -6,9 +6,10 @@ registers and how many workgroups share one column (each recomputing the
 column's rotated samples).  The back
 projector's workgroup stages one sinogram column in LDS; its shape is the
 number of waves (voxel rows in flight) sharing that column.  Results are
-bit-identical for every shape; only speed differs.  The back projector itself
-(direct or incremental) is chosen the same way; those two differ by float
-rounding, as they do in the reference.  The tuner times the
+bit-identical for every shape; only speed differs.  The choice between the
+incremental and the direct back projector is not a shape: the two differ by
+float rounding, as they do in the reference, and that choice stays with the
+caller (hrrt_set_backproject_schedule, the reference's -T).  The tuner times the
 candidates on the caller's own subset update with HIP events and keeps the
 fastest.
 """
@@ -35,32 +36,31 @@ def _time_kernel(g, work, normfac, prompt, add, subsets, repeats, kernel):
 
 
 def tune_backproject(g, image, normfac, prompt, add, subsets=(0, 8), repeats=2, candidates=BP_CANDIDATES):
-    """Choose the back projector: the direct variant at each workgroup size, and the
-    incremental one (every pair, the reference's -T 1 schedule) where the geometry allows it.
-    Returns (choice, {label: ms per back-projection launch}); choice is "incremental" or the
-    direct variant's waves per workgroup.  The two variants differ by float rounding only."""
+    """Choose the direct back projector's workgroup size (waves per workgroup).
+
+    Workgroup sizes give bit-identical results; only speed differs.  The direct projector
+    serves the remainder pairs of a -T schedule, schedule 0 and the precision form, so it
+    is timed under schedule 0.  The caller's back-projection schedule (the reference's
+    -T, which decides per pair between the incremental and the direct projector and with
+    it the float rounding) is restored before returning, and its own back-projection
+    time is reported as "schedule T=<n>".  The tuner never changes the algorithm.
+    Returns (waves, {label: ms per back-projection launch})."""
     work = image.clone()
     results = {}
     keep = g.backproject_schedule
-    g.set_backproject_schedule(0)
-    for w in candidates:
-        g.set_backproject_shape(w)
-        results[f"direct/{w}w"] = _time_kernel(g, work, normfac, prompt, add, subsets, repeats, "backproject")
-    best_w = min(candidates, key=lambda w: results[f"direct/{w}w"])
-    g.set_backproject_shape(best_w)
-    choice = best_w
     try:
-        g.set_backproject_schedule(1)
-    except Exception:
-        g.set_backproject_schedule(keep if keep == 0 else 0)
-    else:
-        results["incremental"] = _time_kernel(g, work, normfac, prompt, add, subsets, repeats, "backproject")
-        if results["incremental"] < results[f"direct/{best_w}w"]:
-            choice = "incremental"
-        else:
-            g.set_backproject_schedule(0)
+        g.set_backproject_schedule(0)
+        for w in candidates:
+            g.set_backproject_shape(w)
+            results[f"direct/{w}w"] = _time_kernel(g, work, normfac, prompt, add, subsets, repeats, "backproject")
+        best_w = min(candidates, key=lambda w: results[f"direct/{w}w"])
+        g.set_backproject_shape(best_w)
+    finally:
+        g.set_backproject_schedule(keep)
+    if keep != 0:
+        results[f"schedule T={keep}"] = _time_kernel(g, work, normfac, prompt, add, subsets, repeats, "backproject")
     del work
-    return choice, results
+    return best_w, results
 
 
 def tune_forward(g, image, normfac, prompt, add, subsets=(0, 8), repeats=2, candidates=CANDIDATES):

@@ -162,6 +162,8 @@ struct hrrt_ctx {
     std::vector<hipGraphExec_t> gexec;   // [subsets]
     int ensure_frame_scratch(int nframes) {
         if (nframes <= f_cap) return HRRT_OK;
+        // instantiated graphs hold the old buffers in their kernel arguments
+        drop_graphs();
         void *s[] = {f_ratio, fA, fB};
         for (void *p : s)
             if (p) (void)hipFree(p);
@@ -635,6 +637,29 @@ extern "C" int hrrt_stage_sinograms(hrrt_ctx *ctx, const int16_t *d_prompt_file,
     return HRRT_OK;
 }
 
+extern "C" int hrrt_stage_sinograms_w012(hrrt_ctx *ctx, int weighting, int precorrected, const int16_t *d_data16_file,
+                                         const float *d_dataf_file, const float *d_delayed_file,
+                                         const float *d_norm_file, const float *d_scatter_file,
+                                         const float *d_atten_file, float *d_data, float *d_invnorm, void *stream) {
+    if (!ctx || !d_data || !d_invnorm) return fail(HRRT_ERR_ARG, "null argument");
+    if (weighting < 0 || weighting > 2) return fail(HRRT_ERR_ARG, "weighting %d is not 0 (UW), 1 (AW) or 2 (ANW)", weighting);
+    if (!d_data16_file == !d_dataf_file) return fail(HRRT_ERR_ARG, "give exactly one of the short and the float data file");
+    if (precorrected && !d_dataf_file) return fail(HRRT_ERR_ARG, "a precorrected sinogram is a float file");
+    if (weighting == 0 && !precorrected && d_scatter_file && !d_norm_file)
+        return fail(HRRT_ERR_UNSUPPORTED, "-W 0 with scatter and no norm: the reference subtracts a buffer it never "
+                                          "filled (hrrt_osem3d_span3.cpp:1233)");
+    const hrrt::Geometry &G = ctx->geo;
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->colmask, sizeof(int) * G.views * G.N))) return r;
+    HIP_TRY(hipMemsetAsync(ctx->colmask, 0, sizeof(int) * G.views * G.N, S(stream)));
+    if (d_norm_file && G.x_rebin * G.v_rebin > 1)
+        if ((r = ctx->zalloc((void **)&ctx->gapmask, (size_t)G.th_pixels * G.p.def_angles * G.p.def_elements))) return r;
+    HIP_TRY(hrrt::launch_stage_w012(ctx->dg, weighting, precorrected != 0, d_data16_file, d_dataf_file, d_delayed_file,
+                                    d_norm_file, d_scatter_file, d_atten_file, d_data, d_invnorm, ctx->gapmask,
+                                    ctx->colmask, S(stream)));
+    return HRRT_OK;
+}
+
 // Back projection of the pairs of a batch under the reference's thread schedule
 // (CalculateNormfac S3:3796-3825, CalculateOsem3d S3:4174-4218): with T threads the first
 // npairs - npairs % T pairs of a subset take the incremental projector (back_proj3d_thread1),
@@ -729,6 +754,39 @@ extern "C" int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_no
     return HRRT_OK;
 }
 
+extern "C" int hrrt_osem_subset_w012(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subset, const float *d_data,
+                                     int isubset, void *stream) {
+    if (!ctx || !d_image || !d_normfac_subset || !d_data) return fail(HRRT_ERR_ARG, "null argument");
+    const hrrt::Geometry &G = ctx->geo;
+    if (isubset < 0 || isubset >= G.p.subsets) return fail(HRRT_ERR_ARG, "subset %d out of range", isubset);
+    int r;
+    if ((r = ctx->ensure_subset_scratch())) return r;
+    if (ctx->psf_on && (r = ctx->ensure_psf_scratch())) return r;
+    hrrt::PairBatch b{isubset, G.p.subsets, ctx->npairs()};
+    ctx->mark(0, S(stream));
+    hrrt::FpIO io;
+    io.add = d_data;
+    const float *fp_image = d_image;
+    if (ctx->psf_on) {
+        HIP_TRY(ctx->convolve3d(d_image, ctx->psf_img, S(stream)));
+        fp_image = ctx->psf_img;
+    }
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, fp_image, ctx->sub_ratio, io, hrrt::FP_RATIO_W012, S(stream)));
+    ctx->mark(1, S(stream));
+    HIP_TRY(backproject_scheduled(ctx, b, ctx->sub_ratio, 0, 0, ctx->bufA, ctx->bufB, S(stream)));
+    ctx->mark(2, S(stream));
+    if (ctx->psf_on) {
+        HIP_TRY(hrrt::launch_rotate_sum(ctx->dg, b, ctx->bufA, ctx->bufB, ctx->psf_corr, S(stream)));
+        HIP_TRY(ctx->convolve3d(ctx->psf_corr, ctx->psf_corr, S(stream)));
+        HIP_TRY(hrrt::launch_update_image(ctx->dg, d_image, ctx->psf_corr, d_normfac_subset, S(stream)));
+    } else {
+        HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->bufA, ctx->bufB, d_image, d_normfac_subset, 0,
+                                           S(stream)));
+    }
+    ctx->mark(-1, S(stream));
+    return HRRT_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Batched dynamic frames: every frame of a study runs the same subset in one
 // launch per kernel (the frame is a grid dimension), frames at fixed strides.
@@ -768,10 +826,9 @@ static int frames_subset_launch(hrrt_ctx *ctx, int nframes, float *d_images, con
         hrrt::forward_shape(ctx->dg, maxt, nsplit, threads, items);
         HIP_TRY(hrrt::launch_ll_reduce(ctx->ll_part, ctx->dg.c * nsplit * b.npairs, nframes, pr.loglik, s));
     }
-    if (pr.mixed)
-        HIP_TRY(hrrt::launch_backproject(ctx->dg, b, ctx->f_ratio, 0, 0, 0, 1, ctx->fA, ctx->fB, s));
-    else
-        HIP_TRY(backproject_scheduled(ctx, b, ctx->f_ratio, 0, 0, ctx->fA, ctx->fB, s));
+    // the subset update back-projects under the schedule in both forms (incremental by default);
+    // the precision form sums the view pairs of the rotation with compensation
+    HIP_TRY(backproject_scheduled(ctx, b, ctx->f_ratio, 0, 0, ctx->fA, ctx->fB, s));
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->fA, ctx->fB, d_images, d_normfac_subset, pr.mixed, s));
     return HRRT_OK;
 }
@@ -1332,6 +1389,12 @@ extern "C" int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float 
                                        void *stream) {
     if (!ctx || !d_est || !d_add || !d_prompt) return fail(HRRT_ERR_ARG, "null argument");
     HIP_TRY(hrrt::launch_ratio_w3(ctx->dg, d_est, d_add, d_prompt, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_update_estimate_w012(hrrt_ctx *ctx, float *d_est, const float *d_data, void *stream) {
+    if (!ctx || !d_est || !d_data) return fail(HRRT_ERR_ARG, "null argument");
+    HIP_TRY(hrrt::launch_ratio_w012(ctx->dg, d_est, d_data, S(stream)));
     return HRRT_OK;
 }
 

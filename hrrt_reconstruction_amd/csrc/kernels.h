@@ -63,10 +63,11 @@ struct PairBatch {
 //   FP_RATIO  : OP ratio with float additive term (bit-exact path)
 //   FP_RATIO16: OP ratio with FP16 additive term; with ll_part set, also the FP64
 //               Poisson log-likelihood of the subset's bins (needs the FP16 1/(N*A))
-enum FpMode { FP_PLAIN = 0, FP_RATIO = 1, FP_RATIO16 = 2 };
+//   FP_RATIO_W012: UW/AW/ANW ratio data / estimate (add holds the corrected data)
+enum FpMode { FP_PLAIN = 0, FP_RATIO = 1, FP_RATIO16 = 2, FP_RATIO_W012 = 3 };
 struct FpIO {
     const int16_t *prompt = nullptr;
-    const void *add = nullptr;           // float (FP_RATIO) or FP16 bits (FP_RATIO16)
+    const void *add = nullptr;           // float (FP_RATIO, FP_RATIO_W012) or FP16 bits (FP_RATIO16)
     const uint16_t *invnorm = nullptr;   // FP16 1/(N*A) (FP_RATIO16 log-likelihood)
     double *ll_part = nullptr;           // one partial sum per workgroup (FP_RATIO16)
 };
@@ -98,6 +99,7 @@ hipError_t launch_rotate_add(const DevGeom &g, int view, const float *bufA, cons
 hipError_t launch_paircol_to_estimate(const DevGeom &g, int view, const float *paircol, float *prj, hipStream_t s);
 hipError_t launch_estimate_to_paircol(const DevGeom &g, float *prj, float *paircol, hipStream_t s);
 hipError_t launch_ratio_w3(const DevGeom &g, float *est, const float *add, const int16_t *prompt, hipStream_t s);
+hipError_t launch_ratio_w012(const DevGeom &g, float *est, const float *data, hipStream_t s);
 hipError_t launch_update_image(const DevGeom &g, float *image, float *corr, const float *nf, hipStream_t s);
 hipError_t launch_normfac_invert(const DevGeom &g, const float *sens, float *out, hipStream_t s);
 // image [X][Y][Z4] <-> flat planes [Z][N][N] (to_flat = 1: image -> flat); flip swaps x and y
@@ -114,6 +116,13 @@ hipError_t launch_end_iteration(const DevGeom &g, float *image, int z_min, int z
 hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *delayed_f, const float *norm_f,
                         const float *scatter_f, const float *atten_f, int16_t *prompt, float *add, float *invnorm,
                         uint8_t *gapmask, int *colmask, hipStream_t s);
+// staging for the UW/AW/ANW weightings (weighting 0, 1, 2): data_f16 (short prompts or trues)
+// or data_ff (float, or fully precorrected when precor) -> data (the w012 numerator) and the
+// sensitivity weights of that weighting
+hipError_t launch_stage_w012(const DevGeom &g, int weighting, int precor, const int16_t *data_f16,
+                             const float *data_ff, const float *delayed_f, const float *norm_f, const float *scatter_f,
+                             const float *atten_f, float *data, float *invnorm, uint8_t *gapmask, int *colmask,
+                             hipStream_t s);
 
 // Records msg as hrrt_last_error()'s message of this thread and returns code.
 int set_last_error(int code, const char *msg);

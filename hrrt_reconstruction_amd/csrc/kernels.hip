@@ -282,7 +282,12 @@ __device__ __forceinline__ void fp_write1(const DevGeom &g, float4 acc, int seg,
         if ((l & 1) && x == g.c) continue;   // at x == N/2 lanes 1/3 duplicate lanes 0/2
         const int col = (l & 1) ? N - x : x, h = l >> 1;
         float v = lane(acc, l);
-        if (MODE != FP_PLAIN) {
+        if (MODE == FP_RATIO_W012) {   // pt_update_est_w012 S3:2436-2447
+            const size_t gi = ((size_t)(view * 2 + h) * N + col) * g.spp + pidx;
+            const float dt = static_cast<const float *>(io.add)[gi];
+            const float q = v > g.eps3 ? dt / v : 0.0f;
+            v = q * resc;
+        } else if (MODE != FP_PLAIN) {
             const size_t gi = ((size_t)(view * 2 + h) * N + col) * g.spp + pidx;
             const float a = MODE == FP_RATIO ? static_cast<const float *>(io.add)[gi]
                                              : half_at(static_cast<const uint16_t *>(io.add), gi);
@@ -326,11 +331,9 @@ __global__ void __launch_bounds__(1024) fp_kernel(const DevGeom g, const PairBat
     const int frame = L0 / (per_pair * b.npairs), L = L0 - frame * per_pair * b.npairs;
     image += frame * b.img_stride / 4;
     out += frame * b.ratio_stride;
-    if (MODE != FP_PLAIN) {
-        io.prompt += frame * b.sino_stride;
-        if (MODE == FP_RATIO) io.add = static_cast<const float *>(io.add) + frame * b.sino_stride;
-        else io.add = static_cast<const uint16_t *>(io.add) + frame * b.sino_stride;
-    }
+    if (MODE == FP_RATIO || MODE == FP_RATIO16) io.prompt += frame * b.sino_stride;
+    if (MODE == FP_RATIO || MODE == FP_RATIO_W012) io.add = static_cast<const float *>(io.add) + frame * b.sino_stride;
+    else if (MODE == FP_RATIO16) io.add = static_cast<const uint16_t *>(io.add) + frame * b.sino_stride;
     double ll = 0.0;
     const int lp = L / per_pair, rem = L - lp * per_pair;
     const int x = g.c - rem / nsplit, split = rem % nsplit;
@@ -1121,6 +1124,13 @@ __global__ void ratio_w3_kernel(const DevGeom g, float *__restrict__ est, const 
     }
 }
 
+__global__ void ratio_w012_kernel(const DevGeom g, float *__restrict__ est, const float *__restrict__ data, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float e = est[i];
+        est[i] = e > g.eps3 ? data[i] / e : 0.0f;
+    }
+}
+
 __global__ void update_image_kernel(const DevGeom g, float *__restrict__ image, float *__restrict__ corr,
                                     const float *__restrict__ nf) {
     const int x = blockIdx.x;
@@ -1176,6 +1186,11 @@ struct StageArgs {
     const uint8_t *gapmask;   // [th][def_angles][def_elements] when rebinning the norm
     int *colmask;             // [views][N]
     const int16_t *plane_seg; // [S]
+    // UW/AW/ANW staging (weighting 0, 1, 2; 3 = OP above): prompt_f holds short prompts or
+    // trues, dataf_f a float sinogram (fully precorrected when precor); data is the output
+    int weighting = 3, precor = 0;
+    const float *dataf_f = nullptr;
+    float *data = nullptr;
 };
 
 __device__ __forceinline__ float rebin_sum(const DevGeom &g, const float *f, size_t fplane, int v, int col) {
@@ -1206,6 +1221,22 @@ __device__ __forceinline__ float rebin_norm(const DevGeom &g, const float *f, si
         out = xr == 0 ? s : out + s;
     }
     if (out > 0.0f) out = 1.0f / out;
+    return out;
+}
+
+// read_flat_short_proj_ori_to_float_array_theta FIO:185-224 with sino_rebin_float FIO:77-104:
+// views summed as shorts (wrapping), then the bins added in float from 0
+__device__ __forceinline__ float rebin_short_f(const DevGeom &g, const int16_t *f, size_t fplane, int v, int col) {
+    const int DE = g.def_elements;
+    const int16_t *p = f + fplane * (size_t)g.def_angles * DE;
+    if (g.x_rebin * g.v_rebin == 1) return (float)p[(size_t)v * DE + col];
+    float out = 0.0f;
+    for (int xr = 0; xr < g.x_rebin; xr++) {
+        int16_t s = p[(size_t)(v * g.v_rebin) * DE + col * g.x_rebin + xr];
+        for (int vr = 1; vr < g.v_rebin; vr++)
+            s = (int16_t)(s + p[(size_t)(v * g.v_rebin + vr) * DE + col * g.x_rebin + xr]);
+        out = out + (float)s;
+    }
     return out;
 }
 
@@ -1269,6 +1300,55 @@ __global__ void stage_kernel(const DevGeom g, StageArgs a) {
         }
         const float d = a.delayed_f ? rebin_sum(g, a.delayed_f, fplane, v, col) : 0.0f;
         const float at = a.atten_f ? rebin_sum(g, a.atten_f, fplane, v, col) : 1.0f;
+        if (nrm != 0.0f) atomicOr(&a.colmask[v * N + col], 1);
+        const int pair = v % (V / 2), h = v / (V / 2);
+        const size_t o = ((size_t)(pair * 2 + h) * N + col) * g.spp + g.seg_data[th] + i;
+        if (a.weighting < 3) {
+            // prepare_osem3dw0 S3:1198-1256, w1 S3:1346-1407 and w2 S3:1505-1546; precorrected data
+            // S3:1309-1323, 1410-1482, 1548-1636; then setnegative2zero simd_operation.cpp:78-96
+            float t;
+            if (!a.precor) {
+                t = a.prompt_f ? rebin_short_f(g, a.prompt_f, fplane, v, col) : rebin_sum(g, a.dataf_f, fplane, v, col);
+                if (a.delayed_f) t = t - d;                               // readsinogram S3:1124-1131
+                const float sc = a.scatter_f ? rebin_sum(g, a.scatter_f, fplane, v, col) : 0.0f;
+                if (a.weighting == 0) {
+                    if (a.norm_f) {
+                        t = t * nrm;                                      // S3:1224, 1244
+                        if (a.scatter_f) t = t - nrm;                     // S3:1233: subtracts the norm buffer
+                    }
+                    if (a.atten_f) t = t * at;                            // S3:1253
+                } else if (a.weighting == 1) {
+                    if (a.scatter_f) {
+                        if (a.norm_f) t = nrm > 0.0f ? t - sc / nrm : 0.0f;   // S3:1380-1382
+                        else t = t - sc;                                      // S3:1389
+                    }
+                    if (a.norm_f) t = t * nrm;                                // S3:1405
+                } else if (a.scatter_f) {
+                    t = nrm > 0.0f ? t - sc / nrm : 0.0f;                     // S3:1535-1537
+                }
+            } else {
+                t = rebin_sum(g, a.dataf_f, fplane, v, col);
+                if (a.weighting == 2) t = nrm > 0.0f ? t / nrm : 0.0f;        // S3:1564-1566
+                if (a.weighting >= 1 && a.atten_f && at > 0.0f) t = t / at;  // divideprojection simd_operation.cpp:34
+            }
+            a.data[o] = t > 0.0f ? t : 0.0f;
+            float inv;
+            if (a.weighting == 0) {                                           // norm_UW S3:756-771
+                inv = nrm > 0.0f ? 1.0f : nrm;
+            } else if (a.weighting == 1) {                                    // norm_AW S3:774-791
+                inv = nrm > 0.0f ? 1.0f / (at < 1.0f ? 1.0f : at) : 0.0f;
+            } else {                                                          // norm_ANW S3:796-821
+                const float am = at < 1.0f ? 1.0f : at;
+                if (nrm > 0.0f) {
+                    const float den = nrm * am;
+                    inv = den > g.eps2 ? 1.0f / den : 0.0f;
+                } else {
+                    inv = nrm;
+                }
+            }
+            a.invnorm[o] = inv;
+            continue;
+        }
         int16_t pr = 0;
         if (a.prompt_f) pr = rebin_short(g, a.prompt_f, fplane, v, col);
         float ad = d * nrm;                                     // mulprojection S3:1715
@@ -1283,9 +1363,6 @@ __global__ void stage_kernel(const DevGeom g, StageArgs a) {
         } else {
             inv = nrm;
         }
-        if (nrm != 0.0f) atomicOr(&a.colmask[v * N + col], 1);
-        const int pair = v % (V / 2), h = v / (V / 2);
-        const size_t o = ((size_t)(pair * 2 + h) * N + col) * g.spp + g.seg_data[th] + i;
         a.prompt[o] = pr;
         a.add[o] = ad;
         a.invnorm[o] = inv;
@@ -1384,6 +1461,8 @@ static hipError_t fp_launch_t(const DevGeom &g, const PairBatch &b, const float 
     const float4 *im = reinterpret_cast<const float4 *>(image);
     if (mode == FP_RATIO)
         hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO, FAST>), grid, block, shm, s, g, b, im, out, io, nsplit);
+    else if (mode == FP_RATIO_W012)
+        hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO_W012, FAST>), grid, block, shm, s, g, b, im, out, io, nsplit);
     else if (mode == FP_RATIO16)
         hipLaunchKernelGGL((fp_kernel<MAXT, ITEMS, FP_RATIO16, FAST>), grid, block, shm, s, g, b, im, out, io, nsplit);
     else
@@ -1530,6 +1609,12 @@ hipError_t launch_ratio_w3(const DevGeom &g, float *est, const float *add, const
     return hipGetLastError();
 }
 
+hipError_t launch_ratio_w012(const DevGeom &g, float *est, const float *data, hipStream_t s) {
+    const size_t n = (size_t)2 * g.S * g.N;
+    hipLaunchKernelGGL(ratio_w012_kernel, dim3(1024), dim3(256), 0, s, g, est, data, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_update_image(const DevGeom &g, float *image, float *corr, const float *nf, hipStream_t s) {
     hipLaunchKernelGGL(update_image_kernel, dim3(g.N), dim3(64, 4), 0, s, g, image, corr, nf);
     return hipGetLastError();
@@ -1584,6 +1669,25 @@ hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *
         hipLaunchKernelGGL(gapmask_kernel, dim3(256, g.th), dim3(256), 0, s, g, norm_f, gapmask);
     }
     StageArgs a{prompt_f, delayed_f, norm_f, scatter_f, atten_f, prompt, add, invnorm, gapmask, colmask, g.plane_seg};
+    hipLaunchKernelGGL(stage_kernel, dim3(8192), dim3(256), 0, s, g, a);
+    hipLaunchKernelGGL(nzmask_kernel, dim3(g.views / 2), dim3(256), 0, s, g, colmask, g.nzmask);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_w012(const DevGeom &g, int weighting, int precor, const int16_t *data_f16,
+                             const float *data_ff, const float *delayed_f, const float *norm_f, const float *scatter_f,
+                             const float *atten_f, float *data, float *invnorm, uint8_t *gapmask, int *colmask,
+                             hipStream_t s) {
+    const int rebin = g.x_rebin * g.v_rebin;
+    if (norm_f && rebin > 1) {
+        hipLaunchKernelGGL(gapmask_kernel, dim3(256, g.th), dim3(256), 0, s, g, norm_f, gapmask);
+    }
+    StageArgs a{data_f16, delayed_f, norm_f, scatter_f, atten_f, nullptr, nullptr, invnorm, gapmask, colmask,
+                g.plane_seg};
+    a.weighting = weighting;
+    a.precor = precor;
+    a.dataf_f = data_ff;
+    a.data = data;
     hipLaunchKernelGGL(stage_kernel, dim3(8192), dim3(256), 0, s, g, a);
     hipLaunchKernelGGL(nzmask_kernel, dim3(g.views / 2), dim3(256), 0, s, g, colmask, g.nzmask);
     return hipGetLastError();

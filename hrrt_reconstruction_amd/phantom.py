@@ -138,6 +138,30 @@ def make_study(proj_file, rings, span, maxdel, def_elements, def_angles, bin_mm,
                 alpha=alpha)
 
 
+def make_geometry_study(g, total_counts, gen, cpu_gen, with_scatter=False, norm=None, atten=None,
+                        sphere_activity=None):
+    """Seeded phantom and file-layout study for an Osem3D context g: the phantom is forward
+    projected with g's own projector at the reconstruction size and spread back over the
+    file's bins when the context rebins (-X 128: each internal bin covers 2 x 2 file bins).
+    cpu_gen seeds the phantom (sphere positions), gen the sinogram noise and factors.
+    Returns (image [X][Y][Z4], study dict of make_study)."""
+    p = g.params
+    rb = g.dims.x_rebin
+    img, _ = make_phantom(g.N, g.Z4, g.Z, p.bin_size * rb, p.plane_sep, p.z_min, p.z_max, cpu_gen, "cuda",
+                          sphere_activity=sphere_activity)
+    est = g.estimate_zeros()
+    proj = torch.zeros((g.S, g.views, g.N), dtype=torch.float32, device="cuda")
+    for v in range(g.views // 2):
+        g.forward_proj3d(img, v, prj=est)
+        proj[:, v, :] = est[: g.S]
+        proj[:, v + g.views // 2, :] = est[g.S:]
+    if rb > 1:
+        proj = proj.repeat_interleave(g.dims.v_rebin, 1).repeat_interleave(rb, 2) / (rb * g.dims.v_rebin)
+    st = make_study(proj, p.rings, p.span, p.maxdel, p.def_elements, p.def_angles, p.bin_size, p.plane_sep,
+                    p.crystal_radius, total_counts, gen, "cuda", with_scatter=with_scatter, norm=norm, atten=atten)
+    return img, st
+
+
 def estimates_to_file(estimates_by_view, S, views, N, device):
     """Assemble per-view-pair estimates [2S][N] into the file layout [S][views][N]."""
     out = torch.zeros((S, views, N), dtype=torch.float32, device=device)

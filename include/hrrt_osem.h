@@ -125,6 +125,26 @@ int hrrt_stage_sinograms(hrrt_ctx *ctx, const int16_t *d_prompt_file, const floa
                          const float *d_norm_file, const float *d_scatter_file, const float *d_atten_file,
                          int16_t *d_prompt, float *d_add, float *d_invnorm, void *stream);
 
+/* prepare_osem3dw0 / w1 / w2 (S3:1190-1639) and prepare_norm's weights for that weighting
+ * (S3:827-1003: norm_UW :756, norm_AW :774, norm_ANW :796), for the reference's -W 0 (UW),
+ * 1 (AW) and 2 (ANW, its default).  The data are d_data16_file (short prompts, -p, or trues,
+ * -t) or d_dataf_file (a float sinogram, -Q; a fully precorrected one with precorrected = 1,
+ * -P); exactly one is given.  d_delayed_file is subtracted when given (prompt - delayed,
+ * readsinogram S3:1066-1134).  Outputs, pair-column layout:
+ *   d_data    float  the numerator of update_estimate_w012, clamped at 0 (setnegative2zero):
+ *                    UW  ((p-d)*N [- N with scatter, as the reference's S3:1233]) * A
+ *                    AW  (p-d - s/N where N > 0, else 0) * N
+ *                    ANW  p-d - s/N where N > 0, else 0
+ *                    precorrected: UW t, AW t/A (A > 0), ANW t/N (0 where N = 0), then /A
+ *   d_invnorm float  UW N > 0 ? 1 : N;  AW N > 0 ? 1/max(A,1) : 0;  ANW as hrrt_stage_sinograms
+ * The norm is read with the end-plane killing and gap mask of hrrt_stage_sinograms, and the
+ * norm zero-column mask is computed the same way.  HRRT_ERR_UNSUPPORTED for -W 0 with scatter
+ * and no norm: the reference then subtracts a buffer it never filled. */
+int hrrt_stage_sinograms_w012(hrrt_ctx *ctx, int weighting, int precorrected, const int16_t *d_data16_file,
+                              const float *d_dataf_file, const float *d_delayed_file, const float *d_norm_file,
+                              const float *d_scatter_file, const float *d_atten_file, float *d_data,
+                              float *d_invnorm, void *stream);
+
 /* ---------------- sensitivity (CalculateNormfac S3:3760-3865) ---------------- */
 /* For every subset: back-project the subset's view pairs of d_invnorm (BP
  * rescale applied on the fly, d_invnorm itself is not modified), rotate and
@@ -142,6 +162,12 @@ int hrrt_init_image(hrrt_ctx *ctx, float *d_image, void *stream);              /
 int hrrt_osem_subset(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subset,
                      const int16_t *d_prompt, const float *d_add, int isubset, void *stream);
 int hrrt_end_iteration(hrrt_ctx *ctx, float *d_image, void *stream);           /* S3:4288-4297           */
+/* The same subset update for the UW/AW/ANW weightings (-W 0, 1, 2): the forward projection
+ * of each pair goes through update_estimate_w012 (S3:1918-1919, 2436-2447),
+ * est = est > eps3 ? data / est : 0, with d_data from hrrt_stage_sinograms_w012, and
+ * d_normfac_subset from hrrt_calculate_normfac of that weighting's d_invnorm. */
+int hrrt_osem_subset_w012(hrrt_ctx *ctx, float *d_image, const float *d_normfac_subset, const float *d_data,
+                          int isubset, void *stream);
 
 /* ---------------- batched dynamic frames (no counterpart in the reference) ----------------
  * The reference reconstructs one frame per process run; a dynamic study runs the
@@ -168,11 +194,14 @@ int hrrt_osem_iteration_frames(hrrt_ctx *ctx, int nframes, float *d_images, cons
  * The reference keeps every sinogram in float32 and sums in float32.  This form
  * stores the per-bin float inputs of the hot loop -- the additive term
  * (d*N + s)*A and the sensitivity weights 1/(N*A) -- as IEEE FP16 (bits in
- * uint16_t, round to nearest even), keeps prompts exact int16, computes in FP32,
- * accumulates the back projection over segment groups and the rotation over view
- * pairs with compensated (Kahan) summation, and uses FP64 only for the Poisson
- * log-likelihood of each subset and the global count moments.  Results are
- * within tolerance of an FP64 projector, not bit-identical to the FP32 path. */
+ * uint16_t, round to nearest even), keeps prompts exact int16 and computes in FP32.
+ * The sensitivity build accumulates its back projection over segment groups with
+ * compensated (Kahan) summation; every form sums the view pairs of the rotation with
+ * compensation; the subset updates back-project under the -T schedule
+ * (hrrt_set_backproject_schedule, incremental by default) as the FP32 path does.  FP64
+ * is used only for the Poisson log-likelihood of each subset and the global count
+ * moments.  Results are within tolerance of an FP64 projector, not bit-identical to
+ * the FP32 path. */
 int hrrt_to_half(const float *d_in, uint16_t *d_out, long long n, void *stream);
 /* CalculateNormfac (S3:3760) from FP16 1/(N*A), Kahan accumulation. */
 int hrrt_calculate_normfac_mixed(hrrt_ctx *ctx, const uint16_t *d_invnorm16, float *d_normfac, void *stream);
@@ -245,12 +274,13 @@ int hrrt_back_proj3d_rotframe(hrrt_ctx *ctx, float *d_prj, float *d_imagebuf, in
 /* One buffer of rotate_image_buf (PJ:1701-1725, pt_rotate3d2): d_ima += rotate(d_imagebuf,
  * view), the bilinear rotation of the back projectors, over the FOV columns of d_ima. */
 int hrrt_rotate_add(hrrt_ctx *ctx, const float *d_imagebuf, float *d_ima, int view, void *stream);
-/* Back-projection schedule of the FP32 subset updates (hrrt_osem_subset, *_frames) and of
+/* Back-projection schedule of the subset updates (hrrt_osem_subset, *_w012, *_frames, *_mixed) and of
  * hrrt_calculate_normfac, the reference's -T nthreads (S3:3796-3825, 4174-4218): the first
  * npairs - npairs % nthreads view pairs of each subset take the incremental projector
  * (back_proj3d_thread1), the rest the direct one (back_proj3d2); 0 = direct for every pair.
  * Default 1 (every pair incremental) when the geometry allows it, else 0.  The precision
- * form always uses the direct projector with compensated sums. */
+ * form's subset updates follow the same schedule; its sensitivity build takes the direct
+ * projector with compensated sums. */
 int hrrt_set_backproject_schedule(hrrt_ctx *ctx, int nthreads);
 int hrrt_get_backproject_schedule(const hrrt_ctx *ctx, int *nthreads);
 /* PSF resolution model, the reference's -B fwhm1,fwhm2,ratio (S3:3327-3330, 4521-4540;
@@ -295,6 +325,9 @@ int hrrt_read_normfac(hrrt_ctx *ctx, const char *path, float *d_normfac, void *s
 /* update_estimate_w3 (S3:2474): est = prompt==0 ? 0 : (est+add > eps3 ? prompt/(est+add) : 1),
  * over the [2S][N] rows of one view pair. */
 int hrrt_update_estimate_w3(hrrt_ctx *ctx, float *d_est, const float *d_add, const int16_t *d_prompt, void *stream);
+/* update_estimate_w012 (S3:2452): est = est > eps3 ? data/est : 0 over the [2S][N] rows of one
+ * view pair (UW, AW and ANW weighting). */
+int hrrt_update_estimate_w012(hrrt_ctx *ctx, float *d_est, const float *d_data, void *stream);
 /* pt_cal_updateimage (S3:2018): image = min(image_max, image*clamp(corr,0.01,100)*normfac); corr = 0. */
 int hrrt_cal_updateimage(hrrt_ctx *ctx, float *d_image, float *d_correction, const float *d_normfac, void *stream);
 /* pt_normfac_calculate (S3:1852): out = |sens|>eps1 ? 1/sens : 0 over the FOV, z < z_pixels. */

@@ -32,6 +32,14 @@
 extern "C" {
 #endif
 
+/* Value type of the projectors, the per-bin / per-voxel steps and the drivers: float in
+ * the oracle (liboracle.so), double in its FP64 build (liboracle64.so, -DOG_REAL=double).
+ * Geometry tables, positions and the staging of float / short files are float in both. */
+#ifndef OG_REAL
+#define OG_REAL float
+#endif
+typedef OG_REAL og_real;
+
 typedef struct {
     int   rings;           /* crystal rings (104 for HRRT)                 scanner_model.cpp:100 */
     float crystal_radius;  /* mm (234.5)                                                        */
@@ -96,31 +104,34 @@ int      og_sequence(int sets, short *order);     /* S3:535 */
  * prj: [2S][N]; only covered bins are written (xr=0 and columns outside the
  * FOV keep their previous content, PJ:1322,1397). */
 long     og_prjbuf_floats(const og_geom *g);
-void     og_forward_proj(const og_geom *g, const float *image, float *prj, int view, float *prjbuf);
+void     og_forward_proj(const og_geom *g, const og_real *image, og_real *prj, int view, og_real *prjbuf);
 /* back_proj3d2 PJ:1457 with back_proj3d_view_n1 PJ:2927 (direct two-pass BP)
  * followed by rotateimage2 into ima (+=).  prj is rescaled in place. */
-void     og_back_proj_direct(const og_geom *g, float *prj, float *ima, int view, float *imagebuf, float *prjbuf);
+void     og_back_proj_direct(const og_geom *g, og_real *prj, og_real *ima, int view, og_real *imagebuf, og_real *prjbuf);
 /* back_proj3d_thread1 PJ:1622 with back_proj3d_view_n1thread PJ:4162
  * (incremental BP) into imagebuf (overwrites the disk voxels).  prj is
  * rescaled in place; prjbuf is used as scratch. */
-void     og_back_proj_incremental(const og_geom *g, float *prj, float *imagebuf, int view, float *prjbuf);
+void     og_back_proj_incremental(const og_geom *g, og_real *prj, og_real *imagebuf, int view, og_real *prjbuf);
 /* rotateimage2 PJ:1137: out[x][y][:] += bilinear(imagebuf, R(-psi)(x,y)) for x in [xs,xe) */
-void     og_rotate_add(const og_geom *g, const float *imagebuf, float *out, int view, int xs, int xe);
+void     og_rotate_add(const og_geom *g, const og_real *imagebuf, og_real *out, int view, int xs, int xe);
 /* rotated-frame BP only (both passes of the direct variant), no rotation */
-void     og_back_proj_direct_rotframe(const og_geom *g, float *prj, float *imagebuf, int view, float *prjbuf);
+void     og_back_proj_direct_rotframe(const og_geom *g, og_real *prj, og_real *imagebuf, int view, og_real *prjbuf);
 
 /* ---- per-bin / per-voxel ---- */
 /* update_estimate_w3 S3:2474 (short prompts, OP weighting), over nplanes*N bins of one view pair.
  * est, add: [2S][N] float; prompt: [2S][N] int16. */
-void     og_ratio_w3(const og_geom *g, float *est, const float *add, const int16_t *prompt);
+void     og_ratio_w3(const og_geom *g, og_real *est, const og_real *add, const int16_t *prompt);
+/* update_estimate_w012 S3:2452 (UW, AW, ANW weighting): est = est > eps3 ? data/est : 0.
+ * est, data: [2S][N] float. */
+void     og_ratio_w012(const og_geom *g, og_real *est, const og_real *data);
 /* pt_cal_updateimage S3:2018 over all x: image, correction, normfac [X][Y][Z4] */
-void     og_update_image(const og_geom *g, float *image, float *correction, const float *normfac);
+void     og_update_image(const og_geom *g, og_real *image, og_real *correction, const og_real *normfac);
 /* pt_normfac_calculate S3:1852: out = |sens|>eps1 ? 1/sens : 0 (z < z_pixels, FOV rows) */
-void     og_normfac_invert(const og_geom *g, const float *sens, float *out);
+void     og_normfac_invert(const og_geom *g, const og_real *sens, og_real *out);
 /* Init_Image S3:3997 */
-void     og_init_image(const og_geom *g, float *image);
+void     og_init_image(const og_geom *g, og_real *image);
 /* end-of-iteration threshold S3:4288 */
-void     og_end_iteration(const og_geom *g, float *image);
+void     og_end_iteration(const og_geom *g, og_real *image);
 
 /* ---- staging (FIO:29-259, S3:796-1056, 1649-1818) ----
  * Inputs are whole flat files already in memory, in file layout
@@ -131,6 +142,17 @@ void     og_read_flat_short(const og_geom *g, const int16_t *file, int16_t *out,
 /* prepare_norm (ANW/OP branch, Atten3D): out = 1/(N*max(A,1)) as in norm_ANW S3:796;
  * also fills g->norm_zero_mask (S3:864-909). */
 void     og_prepare_norm(og_geom *g, const float *norm_file, const float *atten_file, float *out);
+/* prepare_norm for weighting 0 (UW: 1 where N > 0), 1 (AW: 1/max(A,1) where N > 0, else 0)
+ * or 2/3 (ANW, as og_prepare_norm); Atten3D; fills g->norm_zero_mask. */
+void     og_prepare_norm_w(og_geom *g, int weighting, const float *norm_file, const float *atten_file, float *out);
+/* prepare_osem3dw0/1/2 (Atten3D): the corrected data that update_estimate_w012 divides by the
+ * estimate, clamped at 0.  Exactly one of data16_file (short prompts -p or trues -t) and
+ * dataf_file (float sinogram, -Q; the fully precorrected sinogram when precorrected, -P) is
+ * given; delayed_file is subtracted when given.  Returns 0, or -1 for an unsupported input
+ * (weighting 0 with scatter and no norm, whose result the reference leaves to a stale buffer). */
+int      og_prepare_w012(const og_geom *g, int weighting, int precorrected, const int16_t *data16_file,
+                         const float *dataf_file, const float *delayed_file, const float *norm_file,
+                         const float *scatter_file, const float *atten_file, float *data);
 /* prepare_osem3dw3 (Atten3D): add = (d*N + s)*A, prompts zeroed where N==0 */
 void     og_prepare_w3(const og_geom *g, const int16_t *prompt_file, const float *delayed_file,
                        const float *norm_file, const float *scatter_file, const float *atten_file,
@@ -138,7 +160,7 @@ void     og_prepare_w3(const og_geom *g, const int16_t *prompt_file, const float
 
 /* ---- drivers ---- */
 /* Gather one view pair's rows from a [S][views][N] sinogram into [2S][N] (largeprj[v] S3:2149). */
-void     og_gather_pair_f(const og_geom *g, const float *sino, int view, float *out);
+void     og_gather_pair_f(const og_geom *g, const og_real *sino, int view, og_real *out);
 void     og_gather_pair_s(const og_geom *g, const int16_t *sino, int view, int16_t *out);
 /* ---- PSF resolution modelling (-B, psf.cpp) ---- */
 typedef struct {
@@ -152,14 +174,14 @@ og_psf  *og_psf_create(const og_geom *g, float fwhm1, float fwhm2, float ratio);
 void     og_psf_destroy(og_psf *p);
 int      og_psf_kernel(const og_psf *p, int axis, float *out);   /* returns the size */
 /* convolve3d psf.cpp:441-476: out may alias image; buf [X][Y][Z4] scratch */
-void     og_convolve3d(const og_geom *g, const og_psf *p, const float *image, float *out, float *buf);
+void     og_convolve3d(const og_geom *g, const og_psf *p, const og_real *image, og_real *out, og_real *buf);
 /* CalculateNormfac / CalculateOsem3d with -B (S3:3835-3840, 4172-4229): the sensitivity
  * is blurred before inversion; the forward projector reads the blurred image and the
  * correction is blurred before the update */
-void     og_calculate_normfac_psf(const og_geom *g, const og_psf *p, float *invnorm, float *normfac_all,
+void     og_calculate_normfac_psf(const og_geom *g, const og_psf *p, og_real *invnorm, og_real *normfac_all,
                                   int nthreads);
-void     og_osem_subset_psf(const og_geom *g, const og_psf *p, float *image, float *correction,
-                            const float *normfac_subset, const float *add, const int16_t *prompt, int isubset,
+void     og_osem_subset_psf(const og_geom *g, const og_psf *p, og_real *image, og_real *correction,
+                            const og_real *normfac_subset, const og_real *add, const int16_t *prompt, int isubset,
                             int nthreads, int workers);
 
 /* ---- flat image files: planes [z_pixels][N][N] ---- */
@@ -170,12 +192,17 @@ void     og_image_from_flat(const og_geom *g, const float *flat, float *image, i
  * invnorm: [S][views][N] (= 1/(N*A)); it is rescaled in place, as the reference does.
  * nthreads selects the reference's split between the *_thread1 and *2 variants
  * (S3:3796-3825); nthreads<=0 means "direct variant for every pair". */
-void     og_calculate_normfac(const og_geom *g, float *invnorm, float *normfac_all, int nthreads);
+void     og_calculate_normfac(const og_geom *g, og_real *invnorm, og_real *normfac_all, int nthreads);
 /* One OSEM subset update (CalculateOsem3d body S3:4165-4279) on subset isubset.
  * nthreads as above; workers>1 runs the view pairs on that many POSIX threads
  * (the arithmetic does not depend on workers). */
-void     og_osem_subset(const og_geom *g, float *image, float *correction, const float *normfac_subset,
-                        const float *add, const int16_t *prompt, int isubset, int nthreads, int workers);
+void     og_osem_subset(const og_geom *g, og_real *image, og_real *correction, const og_real *normfac_subset,
+                        const og_real *add, const int16_t *prompt, int isubset, int nthreads, int workers);
+
+/* The same subset update with the UW/AW/ANW ratio (update_estimate_w012 S3:1918-1919);
+ * data: [S][views][N] from og_prepare_w012. */
+void     og_osem_subset_w012(const og_geom *g, og_real *image, og_real *correction, const og_real *normfac_subset,
+                             const og_real *data, int isubset, int nthreads, int workers);
 
 /* ---- smoothed randoms (gen_delays), randoms.c ---- */
 #define OD_NHEADS   8

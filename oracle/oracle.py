@@ -64,58 +64,72 @@ class Geom(C.Structure):
 
 
 def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile (make rebuilds it when a source changed)."""
-    subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []) + ["liboracle.so"], check=True)
+    """Compile liboracle.so and liboracle64.so with the committed Makefile (make rebuilds them when a
+    source changed)."""
+    subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []) + ["all"], check=True)
     return _LIB_PATH
 
 
-_lib = None
+_LIB64_PATH = os.path.join(_HERE, "liboracle64.so")
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+def _bind(L, real_ptr_is_double):
+    vp = C.c_void_p
+    G = C.POINTER(Geom)
+    L.og_create.restype = C.POINTER(Geom)
+    L.og_create.argtypes = [C.POINTER(Params)]
+    L.og_destroy.argtypes = [C.POINTER(Geom)]
+    L.og_sequence.argtypes = [C.c_int, _PS]
+    L.og_prjbuf_floats.restype = C.c_long
+    L.og_prjbuf_floats.argtypes = [C.POINTER(Geom)]
+    L.og_forward_proj.argtypes = [G, vp, vp, C.c_int, vp]
+    L.og_back_proj_direct.argtypes = [G, vp, vp, C.c_int, vp, vp]
+    L.og_back_proj_direct_rotframe.argtypes = [G, vp, vp, C.c_int, vp]
+    L.og_back_proj_incremental.argtypes = [G, vp, vp, C.c_int, vp]
+    L.og_rotate_add.argtypes = [G, vp, vp, C.c_int, C.c_int, C.c_int]
+    L.og_ratio_w3.argtypes = [G, vp, vp, vp]
+    L.og_ratio_w012.argtypes = [G, vp, vp]
+    L.og_update_image.argtypes = [G, vp, vp, vp]
+    L.og_normfac_invert.argtypes = [G, vp, vp]
+    L.og_init_image.argtypes = [G, vp]
+    L.og_end_iteration.argtypes = [G, vp]
+    L.og_gather_pair_f.argtypes = [G, vp, C.c_int, vp]
+    L.og_gather_pair_s.argtypes = [G, vp, C.c_int, vp]
+    L.og_calculate_normfac.argtypes = [G, vp, vp, C.c_int]
+    L.og_osem_subset.argtypes = [G, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+    L.og_osem_subset_w012.argtypes = [G, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+    L.og_psf_create.restype = vp
+    L.og_psf_create.argtypes = [G, C.c_float, C.c_float, C.c_float]
+    L.og_psf_destroy.argtypes = [vp]
+    L.og_psf_kernel.restype = C.c_int
+    L.og_psf_kernel.argtypes = [vp, C.c_int, vp]
+    L.og_convolve3d.argtypes = [G, vp, vp, vp, vp]
+    L.og_calculate_normfac_psf.argtypes = [G, vp, vp, vp, C.c_int]
+    L.og_osem_subset_psf.argtypes = [G, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
+    if real_ptr_is_double:
+        return L
+    # staging and flat files: float files, built into the float oracle only
+    L.og_read_flat_float.argtypes = [G, vp, vp, C.c_int, C.c_float, C.c_int, C.c_int]
+    L.og_read_flat_short.argtypes = [G, vp, vp, C.c_int]
+    L.og_prepare_norm.argtypes = [G, vp, vp, vp]
+    L.og_prepare_w3.argtypes = [G, vp, vp, vp, vp, vp, vp, vp]
+    L.og_prepare_norm_w.argtypes = [G, C.c_int, vp, vp, vp]
+    L.og_prepare_w012.restype = C.c_int
+    L.og_prepare_w012.argtypes = [G, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    L.og_image_to_flat.argtypes = [G, vp, vp, C.c_int]
+    L.og_image_from_flat.argtypes = [G, vp, vp, C.c_int]
+    return L
+
+
+def lib(double: bool = False):
+    """The float oracle (liboracle.so), or its FP64 build (liboracle64.so) with double=True."""
+    path = _LIB64_PATH if double else _LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(_LIB_PATH)
-        vp = C.c_void_p
-        L.og_create.restype = C.POINTER(Geom)
-        L.og_create.argtypes = [C.POINTER(Params)]
-        L.og_destroy.argtypes = [C.POINTER(Geom)]
-        L.og_sequence.argtypes = [C.c_int, _PS]
-        L.og_prjbuf_floats.restype = C.c_long
-        L.og_prjbuf_floats.argtypes = [C.POINTER(Geom)]
-        G = C.POINTER(Geom)
-        L.og_forward_proj.argtypes = [G, vp, vp, C.c_int, vp]
-        L.og_back_proj_direct.argtypes = [G, vp, vp, C.c_int, vp, vp]
-        L.og_back_proj_direct_rotframe.argtypes = [G, vp, vp, C.c_int, vp]
-        L.og_back_proj_incremental.argtypes = [G, vp, vp, C.c_int, vp]
-        L.og_rotate_add.argtypes = [G, vp, vp, C.c_int, C.c_int, C.c_int]
-        L.og_ratio_w3.argtypes = [G, vp, vp, vp]
-        L.og_update_image.argtypes = [G, vp, vp, vp]
-        L.og_normfac_invert.argtypes = [G, vp, vp]
-        L.og_init_image.argtypes = [G, vp]
-        L.og_end_iteration.argtypes = [G, vp]
-        L.og_read_flat_float.argtypes = [G, vp, vp, C.c_int, C.c_float, C.c_int, C.c_int]
-        L.og_read_flat_short.argtypes = [G, vp, vp, C.c_int]
-        L.og_prepare_norm.argtypes = [G, vp, vp, vp]
-        L.og_prepare_w3.argtypes = [G, vp, vp, vp, vp, vp, vp, vp]
-        L.og_gather_pair_f.argtypes = [G, vp, C.c_int, vp]
-        L.og_gather_pair_s.argtypes = [G, vp, C.c_int, vp]
-        L.og_calculate_normfac.argtypes = [G, vp, vp, C.c_int]
-        L.og_osem_subset.argtypes = [G, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
-        L.og_image_to_flat.argtypes = [G, vp, vp, C.c_int]
-        L.og_image_from_flat.argtypes = [G, vp, vp, C.c_int]
-        L.og_psf_create.restype = vp
-        L.og_psf_create.argtypes = [G, C.c_float, C.c_float, C.c_float]
-        L.og_psf_destroy.argtypes = [vp]
-        L.og_psf_kernel.restype = C.c_int
-        L.og_psf_kernel.argtypes = [vp, C.c_int, vp]
-        L.og_convolve3d.argtypes = [G, vp, vp, vp, vp]
-        L.og_calculate_normfac_psf.argtypes = [G, vp, vp, vp, C.c_int]
-        L.og_osem_subset_psf.argtypes = [G, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
-        _lib = L
-    return _lib
+        _libs[path] = _bind(C.CDLL(path), double)
+    return _libs[path]
 
 
 def _ptr(a):
@@ -140,10 +154,16 @@ def make_params(**kw) -> Params:
 
 
 class Oracle:
-    """Owns one og_geom and exposes the restated reference routines on numpy arrays."""
+    """Owns one og_geom and exposes the restated reference routines on numpy arrays.
 
-    def __init__(self, **kw):
-        self.L = lib()
+    double=True binds the FP64 build (liboracle64.so): the same routines with double values
+    (image, sinograms, buffers, sums) and the same float geometry, positions and factors.
+    Value arrays must then be float64; staging and flat files exist in the float build only."""
+
+    def __init__(self, double=False, **kw):
+        self.double = bool(double)
+        self.real = np.float64 if self.double else np.float32
+        self.L = lib(self.double)
         self.params = make_params(**kw)
         self.gp = self.L.og_create(C.byref(self.params))
         if not self.gp:
@@ -168,6 +188,14 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+    def _v(self, a):
+        """Pointer to a value array (image, sinogram, buffer) of this build's value type."""
+        if a is None:
+            return None
+        if a.dtype != self.real:
+            raise TypeError(f"value arrays of this oracle are {np.dtype(self.real).name}, not {a.dtype}")
+        return _ptr(a)
 
     # ---- tables ----
     def arr(self, name, n, dtype=None):
@@ -251,13 +279,13 @@ class Oracle:
 
     # ---- buffers ----
     def image_zeros(self):
-        return np.zeros((self.N, self.N, self.Z4), np.float32)
+        return np.zeros((self.N, self.N, self.Z4), self.real)
 
     def prj_zeros(self):
-        return np.zeros((2 * self.S, self.N), np.float32)
+        return np.zeros((2 * self.S, self.N), self.real)
 
     def prjbuf(self):
-        return np.zeros(self.L.og_prjbuf_floats(self.gp), np.float32)
+        return np.zeros(self.L.og_prjbuf_floats(self.gp), self.real)
 
     def sino_zeros(self, dtype=np.float32):
         return np.zeros((self.S, self.views, self.N), dtype)
@@ -266,55 +294,55 @@ class Oracle:
     def forward(self, image, view, prj=None, prjbuf=None):
         prj = self.prj_zeros() if prj is None else prj
         prjbuf = self.prjbuf() if prjbuf is None else prjbuf
-        self.L.og_forward_proj(self.gp, _ptr(image), _ptr(prj), view, _ptr(prjbuf))
+        self.L.og_forward_proj(self.gp, self._v(image), self._v(prj), view, self._v(prjbuf))
         return prj
 
     def back_direct(self, prj, view, ima=None):
         prj = prj.copy()
         ima = self.image_zeros() if ima is None else ima
         ib = self.image_zeros()
-        self.L.og_back_proj_direct(self.gp, _ptr(prj), _ptr(ima), view, _ptr(ib), _ptr(self.prjbuf()))
+        self.L.og_back_proj_direct(self.gp, self._v(prj), self._v(ima), view, self._v(ib), self._v(self.prjbuf()))
         return ima
 
     def back_direct_rotframe(self, prj, view):
         prj = prj.copy()
         ib = self.image_zeros()
-        self.L.og_back_proj_direct_rotframe(self.gp, _ptr(prj), _ptr(ib), view, _ptr(self.prjbuf()))
+        self.L.og_back_proj_direct_rotframe(self.gp, self._v(prj), self._v(ib), view, self._v(self.prjbuf()))
         return ib
 
     def back_incremental_rotframe(self, prj, view):
         prj = prj.copy()
         ib = self.image_zeros()
-        self.L.og_back_proj_incremental(self.gp, _ptr(prj), _ptr(ib), view, _ptr(self.prjbuf()))
+        self.L.og_back_proj_incremental(self.gp, self._v(prj), self._v(ib), view, self._v(self.prjbuf()))
         return ib
 
     def rotate_add(self, imagebuf, view, out=None):
         out = self.image_zeros() if out is None else out
-        self.L.og_rotate_add(self.gp, _ptr(imagebuf), _ptr(out), view, 0, self.N)
+        self.L.og_rotate_add(self.gp, self._v(imagebuf), self._v(out), view, 0, self.N)
         return out
 
     # ---- elementwise ----
     def ratio_w3(self, est, add, prompt):
-        est = est.copy()
-        self.L.og_ratio_w3(self.gp, _ptr(est), _ptr(np.ascontiguousarray(add, np.float32)),
+        est = np.ascontiguousarray(est, self.real).copy()
+        self.L.og_ratio_w3(self.gp, self._v(est), self._v(np.ascontiguousarray(add, self.real)),
                            _ptr(np.ascontiguousarray(prompt, np.int16)))
         return est
 
     def update_image(self, image, correction, normfac):
-        self.L.og_update_image(self.gp, _ptr(image), _ptr(correction), _ptr(normfac))
+        self.L.og_update_image(self.gp, self._v(image), self._v(correction), self._v(normfac))
 
     def normfac_invert(self, sens):
         out = self.image_zeros()
-        self.L.og_normfac_invert(self.gp, _ptr(sens), _ptr(out))
+        self.L.og_normfac_invert(self.gp, self._v(sens), self._v(out))
         return out
 
     def init_image(self):
         img = self.image_zeros()
-        self.L.og_init_image(self.gp, _ptr(img))
+        self.L.og_init_image(self.gp, self._v(img))
         return img
 
     def end_iteration(self, image):
-        self.L.og_end_iteration(self.gp, _ptr(image))
+        self.L.og_end_iteration(self.gp, self._v(image))
 
     # ---- staging ----
     def read_flat_float(self, file, theta, def_val=0.0, norm_flag=0):
@@ -343,24 +371,51 @@ class Oracle:
                              _ptr(scatter_file), _ptr(atten_file), _ptr(prompt), _ptr(add))
         return prompt, add
 
+    def ratio_w012(self, est, data):
+        out = np.ascontiguousarray(est, self.real).copy()
+        self.L.og_ratio_w012(self.gp, self._v(out), self._v(np.ascontiguousarray(data, self.real)))
+        return out
+
+    def prepare_norm_w(self, weighting, norm_file, atten_file):
+        out = self.sino_zeros()
+        self.L.og_prepare_norm_w(self.gp, int(weighting), _ptr(norm_file), _ptr(atten_file), _ptr(out))
+        return out
+
+    def prepare_w012(self, weighting, data_file, delayed_file=None, norm_file=None, scatter_file=None,
+                     atten_file=None, precorrected=False):
+        """data_file: int16 (prompts or trues) or float32 (float or precorrected sinogram)."""
+        out = self.sino_zeros()
+        d16 = data_file if data_file.dtype == np.int16 else None
+        df = data_file if data_file.dtype == np.float32 else None
+        rc = self.L.og_prepare_w012(self.gp, int(weighting), int(bool(precorrected)), _ptr(d16), _ptr(df),
+                                    _ptr(delayed_file), _ptr(norm_file), _ptr(scatter_file), _ptr(atten_file),
+                                    _ptr(out))
+        if rc != 0:
+            raise ValueError("unsupported input combination for prepare_w012")
+        return out
+
     def gather_pair(self, sino, view):
         if sino.dtype == np.int16:
             out = np.zeros((2 * self.S, self.N), np.int16)
             self.L.og_gather_pair_s(self.gp, _ptr(sino), view, _ptr(out))
         else:
             out = self.prj_zeros()
-            self.L.og_gather_pair_f(self.gp, _ptr(sino), view, _ptr(out))
+            self.L.og_gather_pair_f(self.gp, self._v(sino), view, self._v(out))
         return out
 
     # ---- drivers ----
     def calculate_normfac(self, invnorm, nthreads=0):
-        nf = np.zeros((self.subsets, self.N, self.N, self.Z4), np.float32)
-        self.L.og_calculate_normfac(self.gp, _ptr(invnorm), _ptr(nf), nthreads)
+        nf = np.zeros((self.subsets, self.N, self.N, self.Z4), self.real)
+        self.L.og_calculate_normfac(self.gp, self._v(invnorm), self._v(nf), nthreads)
         return nf
 
     def osem_subset(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
-        self.L.og_osem_subset(self.gp, _ptr(image), _ptr(correction), _ptr(normfac_subset), _ptr(add),
-                              _ptr(prompt), isubset, nthreads, workers)
+        self.L.og_osem_subset(self.gp, self._v(image), self._v(correction), self._v(normfac_subset),
+                              self._v(add), _ptr(prompt), isubset, nthreads, workers)
+
+    def osem_subset_w012(self, image, correction, normfac_subset, data, isubset, nthreads=0, workers=1):
+        self.L.og_osem_subset_w012(self.gp, self._v(image), self._v(correction), self._v(normfac_subset),
+                                   self._v(data), isubset, nthreads, workers)
 
     # ---- flat image files ----
     def image_to_flat(self, image, flip=False):
@@ -391,17 +446,17 @@ class Oracle:
     def convolve3d(self, image, out=None):
         out = self.image_zeros() if out is None else out
         buf = self.image_zeros()
-        self.L.og_convolve3d(self.gp, self.psf, _ptr(image), _ptr(out), _ptr(buf))
+        self.L.og_convolve3d(self.gp, self.psf, self._v(image), self._v(out), self._v(buf))
         return out
 
     def calculate_normfac_psf(self, invnorm, nthreads=0):
-        nf = np.zeros((self.subsets, self.N, self.N, self.Z4), np.float32)
-        self.L.og_calculate_normfac_psf(self.gp, self.psf, _ptr(invnorm), _ptr(nf), nthreads)
+        nf = np.zeros((self.subsets, self.N, self.N, self.Z4), self.real)
+        self.L.og_calculate_normfac_psf(self.gp, self.psf, self._v(invnorm), self._v(nf), nthreads)
         return nf
 
     def osem_subset_psf(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
-        self.L.og_osem_subset_psf(self.gp, self.psf, _ptr(image), _ptr(correction), _ptr(normfac_subset),
-                                  _ptr(add), _ptr(prompt), isubset, nthreads, workers)
+        self.L.og_osem_subset_psf(self.gp, self.psf, self._v(image), self._v(correction),
+                                  self._v(normfac_subset), self._v(add), _ptr(prompt), isubset, nthreads, workers)
 
 
 NCRYS = 8 * 2 * 72 * 104          # crystals, [layer][y][head][x]

@@ -1,6 +1,6 @@
 /*
  * osem.c -- oracle restatement of the incremental back projector, the
- * per-bin/per-voxel OSEM steps, sinogram staging and the subset drivers.
+ * per-bin/per-voxel OSEM steps and the subset drivers.
  * TEST INFRASTRUCTURE ONLY (see hrrt_oracle.h).
  */
 #include "hrrt_oracle.h"
@@ -12,7 +12,7 @@
 
 #define PB(g, buf, row, k) ((buf) + ((size_t)(row) * ((g)->th_pixels / 2 + 1) + (k)) * (size_t)(g)->P * 4 + 4)
 
-static void bp_rescale(const og_geom *g, float *prj)
+static void bp_rescale(const og_geom *g, og_real *prj)
 {   /* PJ:1638-1649 */
     const int N = g->N, S = g->segmentsize;
     int k, yr, yr2 = 0, xr;
@@ -31,12 +31,12 @@ static void bp_rescale(const og_geom *g, float *prj)
 /* Incremental back projection: back_proj3d_thread1 PJ:1622-1699 and   */
 /* back_proj3d_view_n1thread PJ:4162-4668.                             */
 /* ------------------------------------------------------------------ */
-static void bp_inc_transpose(const og_geom *g, const float *prj, float *prjbuf)
+static void bp_inc_transpose(const og_geom *g, const og_real *prj, og_real *prjbuf)
 {   /* PJ:1654-1693 */
     const int N = g->N, S = g->segmentsize, K = g->th_pixels / 2;
     int xr, yr, k, l;
     for (xr = 1; xr <= N / 2; xr++) {
-        float *ptr = PB(g, prjbuf, xr, 0);
+        og_real *ptr = PB(g, prjbuf, xr, 0);
         int xr2 = N - xr;
         for (yr = 0; yr < g->z_pixels; yr++) {
             ptr[4 * yr] = prj[(size_t)yr * N + xr];
@@ -47,7 +47,7 @@ static void bp_inc_transpose(const og_geom *g, const float *prj, float *prjbuf)
     }
     for (k = 1; k <= K; k++) {
         for (xr = 1; xr <= N / 2; xr++) {
-            float *ptr = PB(g, prjbuf, xr, k), *ptr2 = PB(g, prjbuf, xr + N / 2, k);
+            og_real *ptr = PB(g, prjbuf, xr, k), *ptr2 = PB(g, prjbuf, xr + N / 2, k);
             int y1 = g->segment_offset[k * 2], y2 = g->segment_offset[k * 2 - 1];
             int y3 = y1 + S, y4 = y2 + S, xr2 = N - xr;
             for (yr = g->yr_bottom[k * 2]; yr <= g->yr_top[k * 2]; yr++, y1++, y2++, y3++, y4++) {
@@ -68,25 +68,25 @@ static void bp_inc_transpose(const og_geom *g, const float *prj, float *prjbuf)
     }
 }
 
-static inline float *voxw(const og_geom *g, float *img, int x, int y)
+static inline og_real *voxw(const og_geom *g, og_real *img, int x, int y)
 {
     return img + ((size_t)x * g->N + y) * g->z4;
 }
 
-static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view, int xs, int xe)
+static void bp_inc_view(const og_geom *g, og_real *image, og_real *prjbuf, int view, int xs, int xe)
 {
     const int N = g->N, c = N / 2, Z = g->z_pixels, Z4 = g->z4, GM = g->groupmax;
-    float *s1 = (float *)calloc((size_t)4 * (Z4 + 2), sizeof(float));
-    float *s2 = (float *)calloc((size_t)4 * (Z4 + 2), sizeof(float));
-    float *d1 = (float *)calloc((size_t)4 * (Z4 + 2), sizeof(float));
-    float *d2 = (float *)calloc((size_t)4 * (Z4 + 2), sizeof(float));
+    og_real *s1 = (og_real *)calloc((size_t)4 * (Z4 + 2), sizeof(og_real));
+    og_real *s2 = (og_real *)calloc((size_t)4 * (Z4 + 2), sizeof(og_real));
+    og_real *d1 = (og_real *)calloc((size_t)4 * (Z4 + 2), sizeof(og_real));
+    og_real *d2 = (og_real *)calloc((size_t)4 * (Z4 + 2), sizeof(og_real));
     int x, y, z, z2, z3, z4, yr, yr2, yr3, yr4, th, th2, l;
 #define LOOP4(stmt) for (l = 0; l < 4; l++) { stmt; }
     for (x = xs; x < xe; x++) {
-        float *pp1[64], *pp2[64];
+        og_real *pp1[64], *pp2[64];
         int ystart = g->cylwiny[2 * x], yend = g->cylwiny[2 * x + 1];
         int mask = 0;
-        float *i1f, *i2f, *i3f, *i4f, *i5f, *i6f, *i7f, *i8f;
+        og_real *i1f, *i2f, *i3f, *i4f, *i5f, *i6f, *i7f, *i8f;
         for (th = 0; th <= g->th_pixels / 2; th++) {
             pp1[th] = PB(g, prjbuf, x, th);
             pp2[th] = PB(g, prjbuf, x + c, th);
@@ -96,9 +96,9 @@ static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view,
         i3f = voxw(g, image, c, x); i4f = voxw(g, image, c, N - x);
         if (g->norm_zero_mask[view * (c + 1) + x] == 0) {
             mask = 1;
-            memset(i1f, 0, sizeof(float) * Z4);
+            memset(i1f, 0, sizeof(og_real) * Z4);
             if (x != c) {
-                memset(i2f, 0, sizeof(float) * Z4); memset(i3f, 0, sizeof(float) * Z4); memset(i4f, 0, sizeof(float) * Z4);
+                memset(i2f, 0, sizeof(og_real) * Z4); memset(i3f, 0, sizeof(og_real) * Z4); memset(i4f, 0, sizeof(og_real) * Z4);
             }
             y = c - 1;
             i1f = voxw(g, image, x, y); i2f = voxw(g, image, N - x, y);
@@ -107,21 +107,21 @@ static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view,
             i7f = voxw(g, image, y, x); i8f = voxw(g, image, y, N - x);
             if (x != c) {
                 if (x == y) {
-                    memset(i1f, 0, sizeof(float) * Z4); memset(i2f, 0, sizeof(float) * Z4);
-                    memset(i5f, 0, sizeof(float) * Z4); memset(i6f, 0, sizeof(float) * Z4);
+                    memset(i1f, 0, sizeof(og_real) * Z4); memset(i2f, 0, sizeof(og_real) * Z4);
+                    memset(i5f, 0, sizeof(og_real) * Z4); memset(i6f, 0, sizeof(og_real) * Z4);
                 } else if (x < y) {
-                    memset(i1f, 0, sizeof(float) * Z4); memset(i2f, 0, sizeof(float) * Z4);
-                    memset(i3f, 0, sizeof(float) * Z4); memset(i4f, 0, sizeof(float) * Z4);
-                    memset(i5f, 0, sizeof(float) * Z4); memset(i6f, 0, sizeof(float) * Z4);
-                    memset(i7f, 0, sizeof(float) * Z4); memset(i8f, 0, sizeof(float) * Z4);
+                    memset(i1f, 0, sizeof(og_real) * Z4); memset(i2f, 0, sizeof(og_real) * Z4);
+                    memset(i3f, 0, sizeof(og_real) * Z4); memset(i4f, 0, sizeof(og_real) * Z4);
+                    memset(i5f, 0, sizeof(og_real) * Z4); memset(i6f, 0, sizeof(og_real) * Z4);
+                    memset(i7f, 0, sizeof(og_real) * Z4); memset(i8f, 0, sizeof(og_real) * Z4);
                 }
             }
             y = c - 2;          /* the y loop below starts here and jumps to the write-out */
         } else {
             /* centre voxels PJ:4265-4285 */
-            memcpy(s1, pp1[0], sizeof(float) * 4 * Z);
+            memcpy(s1, pp1[0], sizeof(og_real) * 4 * Z);
             for (th = 2; th < g->th_pixels; th += 2) {
-                float *p1 = pp1[th / 2], *p2 = pp2[th / 2];
+                og_real *p1 = pp1[th / 2], *p2 = pp2[th / 2];
                 for (yr = g->yr_bottom[th]; yr <= g->yr_top[th]; yr++)
                     LOOP4(s1[4 * yr + l] = s1[4 * yr + l] + (p1[4 * yr + l] + p2[4 * yr + l]));
             }
@@ -134,12 +134,12 @@ static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view,
             }
             /* first row y = c-1: direct interpolation, then convert prjbuf to differences PJ:4287-4335 */
             y = c - 1;
-            memset(d1, 0, sizeof(float) * 4 * (Z4 + 2));
-            memset(d2, 0, sizeof(float) * 4 * (Z4 + 2));
-            memcpy(s1, pp1[0], sizeof(float) * 4 * Z);
-            memcpy(s2, pp1[0], sizeof(float) * 4 * Z);
+            memset(d1, 0, sizeof(og_real) * 4 * (Z4 + 2));
+            memset(d2, 0, sizeof(og_real) * 4 * (Z4 + 2));
+            memcpy(s1, pp1[0], sizeof(og_real) * 4 * Z);
+            memcpy(s2, pp1[0], sizeof(og_real) * 4 * Z);
             for (th = 2, th2 = 0; th < g->th_pixels; th += 2, th2++) {
-                float *p1 = pp1[th / 2], *p2 = pp2[th / 2];
+                og_real *p1 = pp1[th / 2], *p2 = pp2[th / 2];
                 const og_blookup *l1 = &g->blookup[(size_t)y * GM + th2];
                 const og_blookup *l2 = &g->blookup[(size_t)(N - y) * GM + th2];
                 const og_blutacc *bc = &g->blutacc[(size_t)(c + 1) * GM + th2];
@@ -169,10 +169,11 @@ static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view,
         for (; y >= ystart; y--) {
             if (mask) goto write_out;
             for (th = 2, th2 = 0; th < g->th_pixels; th += 2, th2++) {
-                float *p1 = pp1[th / 2], *p2 = pp2[th / 2];
+                og_real *p1 = pp1[th / 2], *p2 = pp2[th / 2];
                 const og_blutacc *ba = &g->blutacc[(size_t)y * GM + th2];
                 const og_blutacc *bb = &g->blutacc[(size_t)(N - y) * GM + th2];
-                float tn, coef1, m1;
+                float tn, coef1;
+                og_real m1;
                 if (g->blutacc_flag[(size_t)y * GM + th2] == 0) continue;
                 tn = -g->sin_theta[th] / g->cos_theta[th];
                 tn = 1 - tn * (float)ba->num - ba->coef_pre;
@@ -242,8 +243,8 @@ static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view,
             if (x != c) {
                 if (x == y) {
                     if (mask) {
-                        memset(i1f, 0, sizeof(float) * Z4); memset(i2f, 0, sizeof(float) * Z4);
-                        memset(i5f, 0, sizeof(float) * Z4); memset(i6f, 0, sizeof(float) * Z4);
+                        memset(i1f, 0, sizeof(og_real) * Z4); memset(i2f, 0, sizeof(og_real) * Z4);
+                        memset(i5f, 0, sizeof(og_real) * Z4); memset(i6f, 0, sizeof(og_real) * Z4);
                     } else {
                         for (z = 0; z < Z; z++) {
                             i1f[z] = s1[4 * z] + s2[4 * z + 2];
@@ -254,10 +255,10 @@ static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view,
                     }
                 } else if (x < y) {
                     if (mask) {
-                        memset(i1f, 0, sizeof(float) * Z4); memset(i2f, 0, sizeof(float) * Z4);
-                        memset(i3f, 0, sizeof(float) * Z4); memset(i4f, 0, sizeof(float) * Z4);
-                        memset(i5f, 0, sizeof(float) * Z4); memset(i6f, 0, sizeof(float) * Z4);
-                        memset(i7f, 0, sizeof(float) * Z4); memset(i8f, 0, sizeof(float) * Z4);
+                        memset(i1f, 0, sizeof(og_real) * Z4); memset(i2f, 0, sizeof(og_real) * Z4);
+                        memset(i3f, 0, sizeof(og_real) * Z4); memset(i4f, 0, sizeof(og_real) * Z4);
+                        memset(i5f, 0, sizeof(og_real) * Z4); memset(i6f, 0, sizeof(og_real) * Z4);
+                        memset(i7f, 0, sizeof(og_real) * Z4); memset(i8f, 0, sizeof(og_real) * Z4);
                     } else {
                         for (z = 0; z < Z; z++) {
                             i1f[z] = s1[4 * z]; i2f[z] = s1[4 * z + 1]; i3f[z] = s1[4 * z + 2]; i4f[z] = s1[4 * z + 3];
@@ -286,7 +287,7 @@ static void bp_inc_view(const og_geom *g, float *image, float *prjbuf, int view,
     free(s1); free(s2); free(d1); free(d2);
 }
 
-void og_back_proj_incremental(const og_geom *g, float *prj, float *imagebuf, int view, float *prjbuf)
+void og_back_proj_incremental(const og_geom *g, og_real *prj, og_real *imagebuf, int view, og_real *prjbuf)
 {
     bp_rescale(g, prj);
     bp_inc_transpose(g, prj, prjbuf);
@@ -296,19 +297,29 @@ void og_back_proj_incremental(const og_geom *g, float *prj, float *imagebuf, int
 /* ------------------------------------------------------------------ */
 /* per-bin and per-voxel steps                                         */
 /* ------------------------------------------------------------------ */
-void og_ratio_w3(const og_geom *g, float *est, const float *add, const int16_t *prompt)
+void og_ratio_w3(const og_geom *g, og_real *est, const og_real *add, const int16_t *prompt)
 {   /* pt_update_est_w3 S3:2493-2505 */
     const float eps3 = g->p.epsilon3;
     size_t i, n = (size_t)2 * g->segmentsize * g->N;
     for (i = 0; i < n; i++) {
-        float den = est[i] + add[i];
+        og_real den = est[i] + add[i];
         if (prompt[i] == 0) est[i] = 0;
         else if (den > eps3) est[i] = prompt[i] / den;
         else est[i] = 1.0f;
     }
 }
 
-void og_update_image(const og_geom *g, float *image, float *correction, const float *normfac)
+void og_ratio_w012(const og_geom *g, og_real *est, const og_real *data)
+{   /* pt_update_est_w012 S3:2436-2447 (UW, AW and ANW weighting) */
+    const float eps3 = g->p.epsilon3;
+    size_t i, n = (size_t)2 * g->segmentsize * g->N;
+    for (i = 0; i < n; i++) {
+        if (est[i] > eps3) est[i] = data[i] / est[i];
+        else est[i] = 0.0f;
+    }
+}
+
+void og_update_image(const og_geom *g, og_real *image, og_real *correction, const og_real *normfac)
 {   /* pt_cal_updateimage S3:2033-2056 */
     const int N = g->N, Z4 = g->z4;
     const float amax = 100.00f, amin = 0.01f, coef = g->p.image_max;
@@ -317,9 +328,9 @@ void og_update_image(const og_geom *g, float *image, float *correction, const fl
         for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++) {
             size_t o = ((size_t)x * N + y) * Z4;
             for (z = 0; z < Z4; z++) {
-                float cr = correction[o + z];
-                float cl = amin > cr ? amin : cr;     /* _mm_max_ps(amin, corr) */
-                float t;
+                og_real cr = correction[o + z];
+                og_real cl = amin > cr ? amin : cr;     /* _mm_max_ps(amin, corr) */
+                og_real t;
                 cl = amax < cl ? amax : cl;           /* _mm_min_ps(amax, .)    */
                 t = image[o + z] * (cl * normfac[o + z]);
                 image[o + z] = coef < t ? coef : t;   /* _mm_min_ps(image_max, .) */
@@ -328,7 +339,7 @@ void og_update_image(const og_geom *g, float *image, float *correction, const fl
         }
 }
 
-void og_normfac_invert(const og_geom *g, const float *sens, float *out)
+void og_normfac_invert(const og_geom *g, const og_real *sens, og_real *out)
 {   /* pt_normfac_calculate S3:1870-1889 */
     const int N = g->N, Z4 = g->z4;
     int x, y, z;
@@ -336,15 +347,15 @@ void og_normfac_invert(const og_geom *g, const float *sens, float *out)
         for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++) {
             size_t o = ((size_t)x * N + y) * Z4;
             for (z = 0; z < g->z_pixels; z++)
-                out[o + z] = fabsf(sens[o + z]) > g->p.epsilon1 ? (1.0f / sens[o + z]) : 0.0f;
+                out[o + z] = (sens[o + z] < 0 ? -sens[o + z] : sens[o + z]) > g->p.epsilon1 ? (1.0f / sens[o + z]) : 0.0f;
         }
 }
 
-void og_init_image(const og_geom *g, float *image)
+void og_init_image(const og_geom *g, og_real *image)
 {   /* Init_Image S3:3997-4005 */
     const int N = g->N, Z4 = g->z4;
     int x, y, z;
-    memset(image, 0, sizeof(float) * (size_t)N * N * Z4);
+    memset(image, 0, sizeof(og_real) * (size_t)N * N * Z4);
     for (x = 0; x < N; x++) {
         if (g->cylwiny[2 * x] == g->cylwiny[2 * x + 1]) continue;
         for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++)
@@ -352,248 +363,28 @@ void og_init_image(const og_geom *g, float *image)
     }
 }
 
-void og_end_iteration(const og_geom *g, float *image)
+void og_end_iteration(const og_geom *g, og_real *image)
 {   /* S3:4288-4297 */
     const int N = g->N, Z4 = g->z4;
     int x, y, z;
     for (x = 0; x < N; x++)
         for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++)
             for (z = g->z_min; z < g->z_max; z++) {
-                float *v = &image[((size_t)x * N + y) * Z4 + z];
+                og_real *v = &image[((size_t)x * N + y) * Z4 + z];
                 if (*v < 1e-16) *v = 0;
             }
 }
 
 /* ------------------------------------------------------------------ */
-/* staging                                                              */
-/* ------------------------------------------------------------------ */
-static int swap_theta(int theta)
-{   /* FIO:116-118 */
-    if (theta % 2 == 1) return theta + 1;
-    if (theta == 0) return 0;
-    return theta - 1;
-}
-
-/* sino_rebin FIO:29-74 for one plane, float */
-static void rebin_plane_f(const og_geom *g, float *tmp, float *out, int norm_flag)
-{
-    const int N = g->N, V = g->views, xrb = g->x_rebin, vrb = g->v_rebin;
-    const int r_size = N * xrb;
-    int v, vr, i, xr;
-    for (v = 0; v < V; v++) {
-        float *dst = tmp + (size_t)v * vrb * r_size;
-        for (vr = 0; vr < vrb; vr++) {
-            float *src = dst + (size_t)vr * r_size;
-            if (!norm_flag) {
-                if (vr == 0) continue;
-                for (i = 0; i < r_size; i++) dst[i] += src[i];
-            } else if (vr == 0) {
-                for (i = 0; i < r_size; i++) if (src[i] > 0.0f) dst[i] = 1.0f / src[i];
-            } else {
-                for (i = 0; i < r_size; i++) if (src[i] > 0.0f) dst[i] += 1.0f / src[i];
-            }
-        }
-    }
-    for (v = 0; v < V; v++) {
-        float *dst = out + (size_t)v * N;
-        const float *src = tmp + (size_t)v * vrb * r_size;
-        for (i = 0; i < N; i++, dst++) {
-            *dst = *src++;
-            for (xr = 1; xr < xrb; xr++) *dst += *src++;
-        }
-    }
-    if (norm_flag)
-        for (i = 0; i < N * V; i++) if (out[i] > 0.0f) out[i] = 1.0f / out[i];
-}
-
-void og_read_flat_float(const og_geom *g, const float *file, float *out, int theta, float def_val, int norm_flag, int have_file)
-{   /* read_flat_float_proj_ori_theta FIO:110-180 */
-    const int N = g->N, V = g->views, rebin = g->x_rebin * g->v_rebin;
-    const long plane_out = (long)N * V, file_plane = (long)N * V * rebin;
-    int plane, nplanes, i;
-    theta = swap_theta(theta);
-    nplanes = g->yr_top[theta] - g->yr_bottom[theta] + 1;
-    if (!have_file) {
-        for (i = 0; i < nplanes * plane_out; i++) out[i] = def_val;
-        return;
-    }
-    file += (size_t)g->seg_offset[theta] * rebin;
-    if (rebin == 1) {
-        for (plane = 0; plane < nplanes; plane++) {
-            if (norm_flag && (plane < g->p.npskip || plane + g->p.npskip >= nplanes))
-                memset(out + plane * file_plane, 0, sizeof(float) * file_plane);
-            else
-                memcpy(out + plane * file_plane, file + plane * file_plane, sizeof(float) * file_plane);
-        }
-    } else {
-        unsigned char *mask = (unsigned char *)calloc((size_t)file_plane, 1);
-        float *buf = (float *)malloc(sizeof(float) * file_plane);
-        for (plane = 0; plane < nplanes; plane++) {
-            memcpy(buf, file + plane * file_plane, sizeof(float) * file_plane);
-            if (norm_flag && (plane < g->p.npskip || plane + g->p.npskip >= nplanes)) {
-                memset(out + plane * plane_out, 0, sizeof(float) * plane_out);
-                continue;
-            }
-            if (norm_flag)
-                for (i = 0; i < file_plane; i++) if (buf[i] > 0) mask[i] = 1;
-            rebin_plane_f(g, buf, out + plane * plane_out, norm_flag);
-        }
-        if (norm_flag) {
-            /* sino_rebin on the mask (FIO:169): view sum in place, then radial sum */
-            const int r_size = N * g->x_rebin;
-            unsigned char *mr = (unsigned char *)calloc((size_t)plane_out, 1);
-            int v, vr, xr;
-            for (v = 0; v < V; v++) {
-                unsigned char *dst = mask + (size_t)v * g->v_rebin * r_size;
-                for (vr = 1; vr < g->v_rebin; vr++)
-                    for (i = 0; i < r_size; i++) dst[i] = (unsigned char)(dst[i] + dst[vr * r_size + i]);
-            }
-            for (v = 0; v < V; v++) {
-                const unsigned char *src = mask + (size_t)v * g->v_rebin * r_size;
-                for (i = 0; i < N; i++) {
-                    unsigned char s = *src++;
-                    for (xr = 1; xr < g->x_rebin; xr++) s = (unsigned char)(s + *src++);
-                    mr[v * N + i] = s;
-                }
-            }
-            for (plane = 0; plane < nplanes; plane++)
-                for (i = 0; i < plane_out; i++) if (mr[i] < rebin) out[plane * plane_out + i] = 0.0f;
-            free(mr);
-        }
-        free(mask); free(buf);
-    }
-}
-
-void og_read_flat_short(const og_geom *g, const int16_t *file, int16_t *out, int theta)
-{   /* read_flat_short_proj_ori_to_short_array_theta FIO:226-259 */
-    const int N = g->N, V = g->views, rebin = g->x_rebin * g->v_rebin;
-    const long plane_out = (long)N * V, file_plane = (long)N * V * rebin;
-    int plane, nplanes, i, v, vr, xr;
-    theta = swap_theta(theta);
-    nplanes = g->yr_top[theta] - g->yr_bottom[theta] + 1;
-    file += (size_t)g->seg_offset[theta] * rebin;
-    if (rebin == 1) {
-        memcpy(out, file, sizeof(int16_t) * nplanes * file_plane);
-        return;
-    }
-    {
-        const int r_size = N * g->x_rebin;
-        int16_t *buf = (int16_t *)malloc(sizeof(int16_t) * file_plane);
-        for (plane = 0; plane < nplanes; plane++) {
-            memcpy(buf, file + plane * file_plane, sizeof(int16_t) * file_plane);
-            for (v = 0; v < V; v++) {
-                int16_t *dst = buf + (size_t)v * g->v_rebin * r_size;
-                for (vr = 1; vr < g->v_rebin; vr++)
-                    for (i = 0; i < r_size; i++) dst[i] = (int16_t)(dst[i] + dst[vr * r_size + i]);
-            }
-            for (v = 0; v < V; v++) {
-                const int16_t *src = buf + (size_t)v * g->v_rebin * r_size;
-                int16_t *dst = out + plane * plane_out + (size_t)v * N;
-                for (i = 0; i < N; i++) {
-                    int16_t s = *src++;
-                    for (xr = 1; xr < g->x_rebin; xr++) s = (int16_t)(s + *src++);
-                    dst[i] = s;
-                }
-            }
-        }
-        free(buf);
-    }
-}
-
-static float *theta_tmp(const og_geom *g)
-{
-    return (float *)malloc(sizeof(float) * (size_t)g->z_pixels * g->views * g->N);
-}
-
-void og_prepare_norm(og_geom *g, const float *norm_file, const float *atten_file, float *out)
-{   /* prepare_norm S3:827-1003 (nFlag, Atten3D, ANW/OP weighting) */
-    const int N = g->N, V = g->views;
-    const long pl = (long)N * V;
-    float *tmp = theta_tmp(g);
-    unsigned char *mask = (unsigned char *)calloc((size_t)pl, 1);
-    int theta, yr, v, xr;
-    for (theta = 0; theta < g->th_pixels; theta++) {
-        int np = g->yr_top[theta] - g->yr_bottom[theta] + 1;
-        og_read_flat_float(g, norm_file, tmp, theta, 1.0f, 1, norm_file != NULL);
-        memcpy(out + (size_t)g->segment_offset[theta] * pl, tmp, sizeof(float) * np * pl);
-    }
-    /* zero-column mask S3:864-909 */
-    for (yr = 0; yr < g->segmentsize; yr++)
-        for (v = 0; v < V; v++)
-            for (xr = 0; xr < N; xr++)
-                if (out[(size_t)yr * pl + (size_t)v * N + xr] != 0) mask[v * N + xr] = 1;
-    for (v = 0; v < V / 2; v++)
-        for (xr = 1; xr <= N / 2; xr++)
-            g->norm_zero_mask[v * (N / 2 + 1) + xr] =
-                (mask[v * N + xr] == 0 && mask[(v + V / 2) * N + xr] == 0 && mask[v * N + N - xr] == 0 &&
-                 mask[(v + V / 2) * N + N - xr] == 0) ? 0 : 1;
-    /* 1/(N*max(A,1)) norm_ANW S3:796-821 */
-    for (theta = 0; theta < g->th_pixels; theta++) {
-        int np = g->yr_top[theta] - g->yr_bottom[theta] + 1;
-        long i;
-        float *o = out + (size_t)g->segment_offset[theta] * pl;
-        og_read_flat_float(g, atten_file, tmp, theta, 1.0f, 0, atten_file != NULL);
-        for (i = 0; i < np * pl; i++) {
-            float a = tmp[i];
-            if (a < 1.0f) a = 1.0f;
-            if (o[i] > 0.0f) {
-                float den = o[i] * a;
-                o[i] = den > g->p.epsilon2 ? 1.0f / den : 0.0f;
-            }
-        }
-    }
-    free(tmp); free(mask);
-}
-
-void og_prepare_w3(const og_geom *g, const int16_t *prompt_file, const float *delayed_file,
-                   const float *norm_file, const float *scatter_file, const float *atten_file,
-                   int16_t *prompt, float *add)
-{   /* prepare_osem3dw3 S3:1666-1770 (short prompts, float randoms, Atten3D) */
-    const long pl = (long)g->N * g->views;
-    float *tmp = theta_tmp(g);
-    int theta;
-    long i;
-    for (theta = 0; theta < g->th_pixels; theta++)
-        og_read_flat_short(g, prompt_file, prompt + (size_t)g->segment_offset[theta] * pl, theta);
-    for (theta = 0; theta < g->th_pixels; theta++)
-        og_read_flat_float(g, delayed_file, add + (size_t)g->segment_offset[theta] * pl, theta, 0.0f, 0,
-                           delayed_file != NULL);
-    for (theta = 0; theta < g->th_pixels; theta++) {
-        int np = g->yr_top[theta] - g->yr_bottom[theta] + 1;
-        float *a = add + (size_t)g->segment_offset[theta] * pl;
-        int16_t *pr = prompt + (size_t)g->segment_offset[theta] * pl;
-        og_read_flat_float(g, norm_file, tmp, theta, 1.0f, 1, norm_file != NULL);
-        for (i = 0; i < np * pl; i++) {
-            a[i] = a[i] * tmp[i];
-            if (tmp[i] == 0.0f) pr[i] = 0;
-        }
-    }
-    if (scatter_file)
-        for (theta = 0; theta < g->th_pixels; theta++) {
-            int np = g->yr_top[theta] - g->yr_bottom[theta] + 1;
-            float *a = add + (size_t)g->segment_offset[theta] * pl;
-            og_read_flat_float(g, scatter_file, tmp, theta, 0.0f, 0, 1);
-            for (i = 0; i < np * pl; i++) a[i] = a[i] + tmp[i];
-        }
-    for (theta = 0; theta < g->th_pixels; theta++) {
-        int np = g->yr_top[theta] - g->yr_bottom[theta] + 1;
-        float *a = add + (size_t)g->segment_offset[theta] * pl;
-        og_read_flat_float(g, atten_file, tmp, theta, 1.0f, 0, atten_file != NULL);
-        for (i = 0; i < np * pl; i++) a[i] = a[i] * tmp[i];
-    }
-    free(tmp);
-}
-
-/* ------------------------------------------------------------------ */
 /* drivers                                                              */
 /* ------------------------------------------------------------------ */
-void og_gather_pair_f(const og_geom *g, const float *sino, int view, float *out)
+void og_gather_pair_f(const og_geom *g, const og_real *sino, int view, og_real *out)
 {   /* largeprj[v][j] = largeprj3[j][v], largeprj[v][j+S] = largeprj3[j][v+views/2] (S3:2149-2154) */
     const int N = g->N, S = g->segmentsize, V = g->views;
     int j;
     for (j = 0; j < S; j++) {
-        memcpy(out + (size_t)j * N, sino + ((size_t)j * V + view) * N, sizeof(float) * N);
-        memcpy(out + (size_t)(j + S) * N, sino + ((size_t)j * V + view + V / 2) * N, sizeof(float) * N);
+        memcpy(out + (size_t)j * N, sino + ((size_t)j * V + view) * N, sizeof(og_real) * N);
+        memcpy(out + (size_t)(j + S) * N, sino + ((size_t)j * V + view + V / 2) * N, sizeof(og_real) * N);
     }
 }
 
@@ -607,13 +398,13 @@ void og_gather_pair_s(const og_geom *g, const int16_t *sino, int view, int16_t *
     }
 }
 
-static void scatter_pair_f(const og_geom *g, float *sino, int view, const float *in)
+static void scatter_pair_f(const og_geom *g, og_real *sino, int view, const og_real *in)
 {
     const int N = g->N, S = g->segmentsize, V = g->views;
     int j;
     for (j = 0; j < S; j++) {
-        memcpy(sino + ((size_t)j * V + view) * N, in + (size_t)j * N, sizeof(float) * N);
-        memcpy(sino + ((size_t)j * V + view + V / 2) * N, in + (size_t)(j + S) * N, sizeof(float) * N);
+        memcpy(sino + ((size_t)j * V + view) * N, in + (size_t)j * N, sizeof(og_real) * N);
+        memcpy(sino + ((size_t)j * V + view + V / 2) * N, in + (size_t)(j + S) * N, sizeof(og_real) * N);
     }
 }
 
@@ -653,11 +444,11 @@ typedef struct {
     const og_geom *g;
     int *views;          /* view of each task */
     int *incremental;    /* 1: *_thread1 variant, 0: *2 variant */
-    float **est, **imagebuf, **prjbuf;
-    const float *add;
+    og_real **est, **imagebuf, **prjbuf;
+    const og_real *add;    /* OP: additive term; UW/AW/ANW: the corrected data (prompt == NULL) */
     const int16_t *prompt;
-    float *invnorm;      /* sensitivity mode: rows rescaled in place */
-    const float *image;
+    og_real *invnorm;      /* sensitivity mode: rows rescaled in place */
+    const og_real *image;
     int sens;
 } pair_job;
 
@@ -667,24 +458,29 @@ static void pair_task(void *vctx, int t)
     const og_geom *g = j->g;
     const size_t n2 = (size_t)2 * g->segmentsize * g->N;
     int view = j->views[t];
-    float *est = j->est[t];
-    memset(j->imagebuf[t], 0, sizeof(float) * (size_t)g->N * g->N * g->z4);
+    og_real *est = j->est[t];
+    memset(j->imagebuf[t], 0, sizeof(og_real) * (size_t)g->N * g->N * g->z4);
     if (j->sens) {
         og_gather_pair_f(g, j->invnorm, view, est);
     } else {
-        float *add = (float *)malloc(sizeof(float) * n2);
-        int16_t *pr = (int16_t *)malloc(sizeof(int16_t) * n2);
+        og_real *add = (og_real *)malloc(sizeof(og_real) * n2);
         og_gather_pair_f(g, j->add, view, add);
-        og_gather_pair_s(g, j->prompt, view, pr);
         og_forward_proj(g, j->image, est, view, j->prjbuf[t]);
-        og_ratio_w3(g, est, add, pr);
-        free(add); free(pr);
+        if (j->prompt) {   /* pt_osem3d_proj_thread1 S3:1915-1924 */
+            int16_t *pr = (int16_t *)malloc(sizeof(int16_t) * n2);
+            og_gather_pair_s(g, j->prompt, view, pr);
+            og_ratio_w3(g, est, add, pr);
+            free(pr);
+        } else {
+            og_ratio_w012(g, est, add);
+        }
+        free(add);
     }
     if (j->incremental[t]) og_back_proj_incremental(g, est, j->imagebuf[t], view, j->prjbuf[t]);
     else og_back_proj_direct_rotframe(g, est, j->imagebuf[t], view, j->prjbuf[t]);
 }
 
-typedef struct { const og_geom *g; float **bufs; int *views; int nbuf; float *out; int workers; } rot_job;
+typedef struct { const og_geom *g; og_real **bufs; int *views; int nbuf; og_real *out; int workers; } rot_job;
 
 static void rot_task(void *vctx, int t)
 {   /* each task owns a range of x and adds all buffers in order */
@@ -696,13 +492,13 @@ static void rot_task(void *vctx, int t)
 
 /* Run the view pairs of subset `isubset`, adding their rotated BPs into acc. */
 static void run_subset_pairs(const og_geom *g, int isubset, int nthreads, int workers, int sens,
-                             float *invnorm, const float *image, const float *add, const int16_t *prompt,
-                             float *acc)
+                             og_real *invnorm, const og_real *image, const og_real *add, const int16_t *prompt,
+                             og_real *acc)
 {
     const int npairs = g->sviews / 2;
     const size_t n2 = (size_t)2 * g->segmentsize * g->N;
     int views[256], inc[256], i;
-    float *est[256], *ib[256], *pb[256];
+    og_real *est[256], *ib[256], *pb[256];
     pair_job job;
     rot_job rj;
     int nfull = 0;
@@ -712,9 +508,9 @@ static void run_subset_pairs(const og_geom *g, int isubset, int nthreads, int wo
     for (i = 0; i < npairs; i++) {
         views[i] = g->vieworder[isubset * g->sviews + i];
         inc[i] = i < nfull;
-        est[i] = (float *)calloc(n2, sizeof(float));
-        ib[i] = (float *)calloc((size_t)g->N * g->N * g->z4, sizeof(float));
-        pb[i] = (float *)calloc((size_t)og_prjbuf_floats(g), sizeof(float));
+        est[i] = (og_real *)calloc(n2, sizeof(og_real));
+        ib[i] = (og_real *)calloc((size_t)g->N * g->N * g->z4, sizeof(og_real));
+        pb[i] = (og_real *)calloc((size_t)og_prjbuf_floats(g), sizeof(og_real));
     }
     job.g = g; job.views = views; job.incremental = inc; job.est = est; job.imagebuf = ib; job.prjbuf = pb;
     job.add = add; job.prompt = prompt; job.invnorm = invnorm; job.image = image; job.sens = sens;
@@ -727,86 +523,63 @@ static void run_subset_pairs(const og_geom *g, int isubset, int nthreads, int wo
     }
 }
 
-void og_calculate_normfac(const og_geom *g, float *invnorm, float *normfac_all, int nthreads)
+void og_calculate_normfac(const og_geom *g, og_real *invnorm, og_real *normfac_all, int nthreads)
 {   /* CalculateNormfac S3:3787-3855 */
     const size_t V = (size_t)g->N * g->N * g->z4;
-    float *sens = (float *)malloc(sizeof(float) * V);
+    og_real *sens = (og_real *)malloc(sizeof(og_real) * V);
     int s;
     for (s = 0; s < g->p.subsets; s++) {
-        memset(sens, 0, sizeof(float) * V);
+        memset(sens, 0, sizeof(og_real) * V);
         run_subset_pairs(g, s, nthreads, nthreads > 0 ? nthreads : 1, 1, invnorm, NULL, NULL, NULL, sens);
-        memset(normfac_all + (size_t)s * V, 0, sizeof(float) * V);
+        memset(normfac_all + (size_t)s * V, 0, sizeof(og_real) * V);
         og_normfac_invert(g, sens, normfac_all + (size_t)s * V);
     }
     free(sens);
 }
 
-void og_osem_subset(const og_geom *g, float *image, float *correction, const float *normfac_subset,
-                    const float *add, const int16_t *prompt, int isubset, int nthreads, int workers)
+void og_osem_subset(const og_geom *g, og_real *image, og_real *correction, const og_real *normfac_subset,
+                    const og_real *add, const int16_t *prompt, int isubset, int nthreads, int workers)
 {   /* CalculateOsem3d S3:4174-4245 */
     run_subset_pairs(g, isubset, nthreads, workers, 0, NULL, image, add, prompt, correction);
     og_update_image(g, image, correction, normfac_subset);
 }
 
-void og_calculate_normfac_psf(const og_geom *g, const og_psf *p, float *invnorm, float *normfac_all, int nthreads)
+void og_osem_subset_w012(const og_geom *g, og_real *image, og_real *correction, const og_real *normfac_subset,
+                         const og_real *data, int isubset, int nthreads, int workers)
+{   /* CalculateOsem3d S3:4174-4245 with update_estimate_w012 (-W 0, 1, 2) */
+    run_subset_pairs(g, isubset, nthreads, workers, 0, NULL, image, data, NULL, correction);
+    og_update_image(g, image, correction, normfac_subset);
+}
+
+void og_calculate_normfac_psf(const og_geom *g, const og_psf *p, og_real *invnorm, og_real *normfac_all, int nthreads)
 {   /* CalculateNormfac with -B: convolve3d(normfac, normfac) before pt_normfac_calculate S3:3835-3840 */
     const size_t V = (size_t)g->N * g->N * g->z4;
-    float *sens = (float *)malloc(sizeof(float) * V);
-    float *buf = (float *)calloc(V, sizeof(float));
+    og_real *sens = (og_real *)malloc(sizeof(og_real) * V);
+    og_real *buf = (og_real *)calloc(V, sizeof(og_real));
     int s;
     for (s = 0; s < g->p.subsets; s++) {
-        memset(sens, 0, sizeof(float) * V);
+        memset(sens, 0, sizeof(og_real) * V);
         run_subset_pairs(g, s, nthreads, nthreads > 0 ? nthreads : 1, 1, invnorm, NULL, NULL, NULL, sens);
         og_convolve3d(g, p, sens, sens, buf);
-        memset(normfac_all + (size_t)s * V, 0, sizeof(float) * V);
+        memset(normfac_all + (size_t)s * V, 0, sizeof(og_real) * V);
         og_normfac_invert(g, sens, normfac_all + (size_t)s * V);
     }
     free(buf);
     free(sens);
 }
 
-void og_osem_subset_psf(const og_geom *g, const og_psf *p, float *image, float *correction,
-                        const float *normfac_subset, const float *add, const int16_t *prompt, int isubset,
+void og_osem_subset_psf(const og_geom *g, const og_psf *p, og_real *image, og_real *correction,
+                        const og_real *normfac_subset, const og_real *add, const int16_t *prompt, int isubset,
                         int nthreads, int workers)
 {   /* CalculateOsem3d with -B: image_psf for the forward projector S3:4172-4202, the
      * correction blurred before the update S3:4229 */
     const size_t V = (size_t)g->N * g->N * g->z4;
-    float *image_psf = (float *)calloc(V, sizeof(float));
-    float *buf = (float *)calloc(V, sizeof(float));
+    og_real *image_psf = (og_real *)calloc(V, sizeof(og_real));
+    og_real *buf = (og_real *)calloc(V, sizeof(og_real));
     og_convolve3d(g, p, image, image_psf, buf);
     run_subset_pairs(g, isubset, nthreads, workers, 0, NULL, image_psf, add, prompt, correction);
     og_convolve3d(g, p, correction, correction, buf);
     og_update_image(g, image, correction, normfac_subset);
     free(buf);
     free(image_psf);
-}
-
-/* ------------------------------------------------------------------ */
-/* Flat image files                                                    */
-/* ------------------------------------------------------------------ */
-void og_image_to_flat(const og_geom *g, const float *image, float *flat, int flip)
-{   /* SaveImage S3:4023-4035: tmpimage[z][y][x] = image[x][y][z] (image[y][x][z] flipped), FOV only */
-    const int N = g->N, Z4 = g->z4;
-    int x, y, z;
-    memset(flat, 0, sizeof(float) * (size_t)g->z_pixels * N * N);
-    for (x = 0; x < N; x++)
-        for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++)
-            for (z = 0; z < g->z_pixels; z++)
-                flat[((size_t)z * N + y) * N + x] = flip ? image[((size_t)y * N + x) * Z4 + z]
-                                                         : image[((size_t)x * N + y) * Z4 + z];
-}
-
-void og_image_from_flat(const og_geom *g, const float *flat, float *image, int flip)
-{   /* Init_Image -I S3:3970-3993: tmp[x][y] = plane z; image[y][x][z] = tmp[x][y] (image[x][y][z]
-     * flipped), FOV only; the rest of the image is 0 here (the reference leaves it unset) */
-    const int N = g->N, Z4 = g->z4;
-    int x, y, z;
-    memset(image, 0, sizeof(float) * (size_t)N * N * Z4);
-    for (z = 0; z < g->z_pixels; z++)
-        for (x = 0; x < N; x++)
-            for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++) {
-                const float v = flat[((size_t)z * N + x) * N + y];
-                if (flip) image[((size_t)x * N + y) * Z4 + z] = v;
-                else image[((size_t)y * N + x) * Z4 + z] = v;
-            }
 }

@@ -8,6 +8,11 @@
  *
  * The reference works on float4 "lanes"; here a lane is an explicit index
  * 0..3 and every float operation is issued in the reference's order.
+ *
+ * Values (image, sinogram, buffers, sums) are og_real: float in the oracle,
+ * double in its FP64 build.  Positions, axial fractions and scale factors stay
+ * float in both, so the FP64 build applies the same discretisation (the same
+ * system matrix) and differs only by the rounding of the arithmetic.
  */
 #include "hrrt_oracle.h"
 
@@ -27,46 +32,48 @@ long og_prjbuf_floats(const og_geom *g)
     return (long)(g->N + 1) * (g->th_pixels / 2 + 1) * g->P * 4;
 }
 
-static inline const float *vox(const og_geom *g, const float *img, int x, int y)
+static inline const og_real *vox(const og_geom *g, const og_real *img, int x, int y)
 {
     return img + ((size_t)x * g->N + y) * g->z4;
 }
 
 /* one bilinear sample of the full z column at the rotated position of (xx,yy)
  * (PJ:1821-1828, 1865-1892).  Returns 0 when the sample is rejected. */
-static int sample_column(const og_geom *g, const float *image, float cosv, float sinv,
-                         int xx, int yy, float *out)
+static int sample_column(const og_geom *g, const og_real *image, float cosv, float sinv,
+                         int xx, int yy, og_real *out)
 {
     const int c = g->N / 2, N = g->N;
     float xx0 = (xx - c) * cosv - (yy - c) * sinv + c;
     float yy0 = (xx - c) * sinv + (yy - c) * cosv + c;
     int xi, yi, z;
-    float x1, y1, w1, w2, w3, w4;
-    const float *r1, *r2, *r3, *r4;
+    float x1, y1;
+    og_real w1, w2, w3, w4;
+    const og_real *r1, *r2, *r3, *r4;
     if (xx0 < 0 || xx0 > N - 2 || yy0 < 0 || xx0 > N - 2) return 0;
     xi = (int)xx0; yi = (int)yy0;
     x1 = xx0 - xi; y1 = yy0 - yi;
     r1 = vox(g, image, xi, yi);     r2 = vox(g, image, xi + 1, yi);
     r3 = vox(g, image, xi, yi + 1); r4 = vox(g, image, xi + 1, yi + 1);
-    w1 = (1 - x1) * (1 - y1); w2 = x1 * (1 - y1); w3 = y1 * (1 - x1); w4 = y1 * x1;
+    w1 = ((og_real)1 - x1) * ((og_real)1 - y1); w2 = (og_real)x1 * ((og_real)1 - y1);
+    w3 = (og_real)y1 * ((og_real)1 - x1); w4 = (og_real)y1 * x1;
     for (z = 0; z < g->z4; z++)
         out[z] = (r1[z] * w1 + r2[z] * w2) + (r3[z] * w3 + r4[z] * w4);
     return 1;
 }
 
-static void fp_view(const og_geom *g, const float *image, float *prjbuf, int view, int xs, int xe)
+static void fp_view(const og_geom *g, const og_real *image, og_real *prjbuf, int view, int xs, int xe)
 {
     const int N = g->N, c = N / 2, Z = g->z_pixels, Z4 = g->z4, ZS = g->zshift;
     const int L = g->groupmax * 2 * ZS;
     const float cosv = g->cos_psi[view], sinv = g->sin_psi[view];
-    float *rim = (float *)calloc((size_t)8 * Z4, sizeof(float));
+    og_real *rim = (og_real *)calloc((size_t)8 * Z4, sizeof(og_real));
     /* pam/pam2: float4 per plane, one zero pad plane beyond z_pixels */
-    float *pam = (float *)calloc((size_t)(Z4 + 2) * 4, sizeof(float));
-    float *pam2 = (float *)calloc((size_t)(Z4 + 2) * 4, sizeof(float));
+    og_real *pam = (og_real *)calloc((size_t)(Z4 + 2) * 4, sizeof(og_real));
+    og_real *pam2 = (og_real *)calloc((size_t)(Z4 + 2) * 4, sizeof(og_real));
     int x, y, i, z, z2, l, th, th2, yr, yri;
 
     for (x = xs; x < xe; x++) {
-        float *pp1[64], *pp2[64];
+        og_real *pp1[64], *pp2[64];
         if (x == 0) continue;
         for (i = 0; i <= g->th_pixels / 2; i++) {
             pp1[i] = PB(g, prjbuf, x, i);
@@ -102,7 +109,7 @@ static void fp_view(const og_geom *g, const float *image, float *prjbuf, int vie
             const og_flookup *lp2 = &g->flookup[(size_t)(N - y) * L];
             for (i = 0; i < 8; i++)
                 if (!sample_column(g, image, cosv, sinv, sx[i], sy[i], rim + i * Z4))
-                    memset(rim + i * Z4, 0, sizeof(float) * Z4);
+                    memset(rim + i * Z4, 0, sizeof(og_real) * Z4);
             for (z = 0; z < Z; z++)
                 for (l = 0; l < 4; l++) {
                     pam[4 * z + l] = rim[l * Z4 + z];
@@ -114,7 +121,7 @@ static void fp_view(const og_geom *g, const float *image, float *prjbuf, int vie
                     pp1[0][4 * z + l] = pp1[0][4 * z + l] + (pam[4 * z + l] + pam2[4 * z + l]);
             /* oblique segments PJ:1916-1951 */
             for (th = 2, th2 = 0; th < g->th_pixels; th += 2, th2++) {
-                float *p1 = pp1[th / 2], *p2 = pp2[th / 2];
+                og_real *p1 = pp1[th / 2], *p2 = pp2[th / 2];
                 float coef1, coef2, zz0;
                 yri = th2 * ZS * 2;
                 if (lp1[yri].zzi < 0) continue;
@@ -160,14 +167,14 @@ static void fp_view(const og_geom *g, const float *image, float *prjbuf, int vie
     free(rim); free(pam); free(pam2);
 }
 
-void og_forward_proj(const og_geom *g, const float *image, float *prj, int view, float *prjbuf)
+void og_forward_proj(const og_geom *g, const og_real *image, og_real *prj, int view, og_real *prjbuf)
 {
     const int N = g->N, S = g->segmentsize, K = g->th_pixels / 2;
     int xr, xri, yr, k, y1, y2;
     fp_view(g, image, prjbuf, view, 0, N / 2 + 1);
     /* scatter into [2S][N] PJ:1397-1436 */
     for (xr = 1; xr <= N / 2; xr++) {
-        const float *ptr = PB(g, prjbuf, xr, 0);
+        const og_real *ptr = PB(g, prjbuf, xr, 0);
         if (g->cylwiny[2 * xr] == 0) continue;
         xri = N - xr;
         for (yr = 0; yr < g->z_pixels; yr++) {
@@ -179,7 +186,7 @@ void og_forward_proj(const og_geom *g, const float *image, float *prj, int view,
     }
     for (k = 1; k <= K; k++) {
         for (xr = 0; xr <= N / 2; xr++) {
-            const float *ptr;
+            const og_real *ptr;
             if (g->cylwiny[2 * xr] == 0) continue;
             xri = N - xr;
             ptr = PB(g, prjbuf, xr, k);
@@ -203,7 +210,7 @@ void og_forward_proj(const og_geom *g, const float *image, float *prj, int view,
 }
 
 /* in-place rescale of both views' rows by cos^3(theta)*span*ring_spacing/(2R) (PJ:1496-1507) */
-static void bp_rescale(const og_geom *g, float *prj)
+static void bp_rescale(const og_geom *g, og_real *prj)
 {
     const int N = g->N, S = g->segmentsize;
     int k, yr, yr2 = 0, xr;
@@ -220,12 +227,12 @@ static void bp_rescale(const og_geom *g, float *prj)
 }
 
 /* transpose one view of the pair into prjbuf lanes for the direct BP (PJ:1511-1540, 1556-1583) */
-static void bp_direct_transpose(const og_geom *g, const float *prj, float *prjbuf, int base)
+static void bp_direct_transpose(const og_geom *g, const og_real *prj, og_real *prjbuf, int base)
 {
     const int N = g->N, K = g->th_pixels / 2;
     int xr, yr, k, y1, y2;
     for (xr = 1; xr <= N / 2; xr++) {
-        float *ptr = PB(g, prjbuf, xr, 0);
+        og_real *ptr = PB(g, prjbuf, xr, 0);
         for (yr = 0; yr < g->z_pixels; yr++) {
             ptr[4 * yr] = prj[(size_t)(base + yr) * N + xr];
             ptr[4 * yr + 2] = prj[(size_t)(base + yr) * N + N - xr];
@@ -234,7 +241,7 @@ static void bp_direct_transpose(const og_geom *g, const float *prj, float *prjbu
     }
     for (k = 1; k <= K; k++) {
         for (xr = 1; xr <= N / 2; xr++) {
-            float *ptr = PB(g, prjbuf, xr, k);
+            og_real *ptr = PB(g, prjbuf, xr, k);
             int l;
             y1 = g->segment_offset[k * 2] + base;
             y2 = g->segment_offset[k * 2 - 1] + base;
@@ -253,32 +260,32 @@ static void bp_direct_transpose(const og_geom *g, const float *prj, float *prjbu
     }
 }
 
-static inline float *voxw(const og_geom *g, float *img, int x, int y)
+static inline og_real *voxw(const og_geom *g, og_real *img, int x, int y)
 {
     return img + ((size_t)x * g->N + y) * g->z4;
 }
 
 /* back_proj3d_view_n1 PJ:2927-3086 for x in [xs, xe) */
-static void bp_direct_pass(const og_geom *g, float *image, float *prjbuf, int f90, int view, int xs, int xe)
+static void bp_direct_pass(const og_geom *g, og_real *image, og_real *prjbuf, int f90, int view, int xs, int xe)
 {
     const int N = g->N, c = N / 2, Z = g->z_pixels, Z4 = g->z4;
-    float *i1 = (float *)malloc(sizeof(float) * 4 * Z4);
-    float *i2 = (float *)malloc(sizeof(float) * 4 * Z4);
+    og_real *i1 = (og_real *)malloc(sizeof(og_real) * 4 * Z4);
+    og_real *i2 = (og_real *)malloc(sizeof(og_real) * 4 * Z4);
     int x, y, z, z2, yr, yr2, th, th2;
     for (x = xs; x < xe; x++) {
-        const float *p0 = PB(g, prjbuf, x, 0);
+        const og_real *p0 = PB(g, prjbuf, x, 0);
         int ystart = g->cylwiny[2 * x], yend = g->cylwiny[2 * x + 1];
         int mask = g->norm_zero_mask[view * (c + 1) + x] == 0;
         if (ystart == 0 && yend == 0) continue;
         for (y = ystart; y < c; y++) {
-            float *o1, *o2, *o3, *o4;
+            og_real *o1, *o2, *o3, *o4;
             if (!mask) {
                 const og_blookup *l1 = &g->blookup[(size_t)y * g->groupmax];
                 const og_blookup *l2 = &g->blookup[(size_t)(N - y) * g->groupmax];
-                memcpy(i1, p0, sizeof(float) * 4 * Z);
-                memcpy(i2, p0, sizeof(float) * 4 * Z);
+                memcpy(i1, p0, sizeof(og_real) * 4 * Z);
+                memcpy(i2, p0, sizeof(og_real) * 4 * Z);
                 for (th = 2, th2 = 0; th < g->th_pixels; th += 2, th2++) {
-                    const float *p = PB(g, prjbuf, x, th / 2);
+                    const og_real *p = PB(g, prjbuf, x, th / 2);
                     float coef1 = l1[th2].yr2a, coef2 = l2[th2].yr2a;
                     int l;
                     z2 = l2[th2].zsa; yr2 = l2[th2].yra;
@@ -293,8 +300,8 @@ static void bp_direct_pass(const og_geom *g, float *image, float *prjbuf, int f9
                 o1 = voxw(g, image, x, y); o2 = voxw(g, image, x, N - y);
                 o3 = voxw(g, image, N - x, y); o4 = voxw(g, image, N - x, N - y);
                 if (mask) {
-                    memset(o1, 0, sizeof(float) * Z4); memset(o2, 0, sizeof(float) * Z4);
-                    if (x != c) { memset(o3, 0, sizeof(float) * Z4); memset(o4, 0, sizeof(float) * Z4); }
+                    memset(o1, 0, sizeof(og_real) * Z4); memset(o2, 0, sizeof(og_real) * Z4);
+                    if (x != c) { memset(o3, 0, sizeof(og_real) * Z4); memset(o4, 0, sizeof(og_real) * Z4); }
                 } else {
                     for (z = 0; z < Z; z++) {
                         o1[z] = i1[4 * z] + i2[4 * z + 1];
@@ -319,20 +326,20 @@ static void bp_direct_pass(const og_geom *g, float *image, float *prjbuf, int f9
             }
         }
         /* centre row PJ:3048-3083 (not masked in the reference) */
-        memcpy(i1, p0, sizeof(float) * 4 * Z);
+        memcpy(i1, p0, sizeof(og_real) * 4 * Z);
         for (th = 2; th < g->th_pixels; th += 2) {
-            const float *p = PB(g, prjbuf, x, th / 2);
+            const og_real *p = PB(g, prjbuf, x, th / 2);
             int l;
             for (z = g->yr_bottom[th]; z <= g->yr_top[th]; z++)
                 for (l = 0; l < 4; l++) i1[4 * z + l] = i1[4 * z + l] + p[4 * z + l];
         }
         {
-            float *o1, *o4;
+            og_real *o1, *o4;
             if (f90 == 0) { o1 = voxw(g, image, x, c); o4 = voxw(g, image, N - x, c); }
             else { o1 = voxw(g, image, c, x); o4 = voxw(g, image, c, N - x); }
             for (z = 0; z < Z; z++) {
-                float a = i1[4 * z] + i1[4 * z + 1];
-                float b = i1[4 * z + 2] + i1[4 * z + 3];
+                og_real a = i1[4 * z] + i1[4 * z + 1];
+                og_real b = i1[4 * z + 2] + i1[4 * z + 3];
                 if (f90 == 0) {
                     o1[z] = a;
                     if (x != c) o4[z] = b;
@@ -346,7 +353,7 @@ static void bp_direct_pass(const og_geom *g, float *image, float *prjbuf, int f9
     free(i1); free(i2);
 }
 
-void og_back_proj_direct_rotframe(const og_geom *g, float *prj, float *imagebuf, int view, float *prjbuf)
+void og_back_proj_direct_rotframe(const og_geom *g, og_real *prj, og_real *imagebuf, int view, og_real *prjbuf)
 {
     const int N = g->N;
     bp_rescale(g, prj);
@@ -356,13 +363,13 @@ void og_back_proj_direct_rotframe(const og_geom *g, float *prj, float *imagebuf,
     bp_direct_pass(g, imagebuf, prjbuf, 1, view, 0, N / 2 + 1);
 }
 
-void og_back_proj_direct(const og_geom *g, float *prj, float *ima, int view, float *imagebuf, float *prjbuf)
+void og_back_proj_direct(const og_geom *g, og_real *prj, og_real *ima, int view, og_real *imagebuf, og_real *prjbuf)
 {
     og_back_proj_direct_rotframe(g, prj, imagebuf, view, prjbuf);
     og_rotate_add(g, imagebuf, ima, view, 0, g->N);
 }
 
-void og_rotate_add(const og_geom *g, const float *imagebuf, float *out, int view, int xs, int xe)
+void og_rotate_add(const og_geom *g, const og_real *imagebuf, og_real *out, int view, int xs, int xe)
 {   /* rotateimage2 PJ:1137-1186 (zoom == 1) */
     const int N = g->N, Z4 = g->z4;
     const float cosv = g->cos_psi[view] / 1.0f, sinv = -g->sin_psi[view] / 1.0f;
@@ -375,9 +382,10 @@ void og_rotate_add(const og_geom *g, const float *imagebuf, float *out, int view
         float yy0 = yyy0 + ystart * cosv + (float)(unsigned)x * sinv;
         for (y = ystart; y < yend; y++) {
             int xx = (int)xx0, yy = (int)yy0;
-            float x1, y1, w1, w2, w3, w4;
-            const float *im1, *im2, *im3, *im4;
-            float *o;
+            float x1, y1;
+            og_real w1, w2, w3, w4;
+            const og_real *im1, *im2, *im3, *im4;
+            og_real *o;
             if (xx < 0 || xx > N - 2 || yy < 0 || yy > N - 2) {
                 xx0 -= sinv; yy0 += cosv;
                 continue;
@@ -386,7 +394,8 @@ void og_rotate_add(const og_geom *g, const float *imagebuf, float *out, int view
             im1 = vox(g, imagebuf, xx, yy); im2 = vox(g, imagebuf, xx + 1, yy);
             im3 = vox(g, imagebuf, xx, yy + 1); im4 = vox(g, imagebuf, xx + 1, yy + 1);
             o = out + ((size_t)x * N + y) * Z4;
-            w1 = (1 - x1) * (1 - y1); w2 = x1 * (1 - y1); w3 = y1 * (1 - x1); w4 = y1 * x1;
+            w1 = ((og_real)1 - x1) * ((og_real)1 - y1); w2 = (og_real)x1 * ((og_real)1 - y1);
+            w3 = (og_real)y1 * ((og_real)1 - x1); w4 = (og_real)y1 * x1;
             for (z = 0; z < Z4; z++)
                 o[z] = o[z] + ((im1[z] * w1 + im2[z] * w2) + (im3[z] * w3 + im4[z] * w4));
             xx0 -= sinv; yy0 += cosv;

@@ -98,10 +98,10 @@ int og_psf_kernel(const og_psf *p, int axis, float *out)
 }
 
 /* convolve3d psf.cpp:441-476: x pass image -> buf, then y/z pass buf -> out (out may be image) */
-void og_convolve3d(const og_geom *g, const og_psf *p, const float *image, float *out, float *buf)
+void og_convolve3d(const og_geom *g, const og_psf *p, const og_real *image, og_real *out, og_real *buf)
 {
     const int N = g->N, Z4 = g->z4;
-    float *acc = (float *)malloc(sizeof(float) * (size_t)Z4);
+    og_real *acc = (og_real *)malloc(sizeof(og_real) * (size_t)Z4);
     int x, y, z, mask;
     {   /* convolvex psf.cpp:168-219 */
         const float *f = p->k[0];
@@ -110,11 +110,11 @@ void og_convolve3d(const og_geom *g, const og_psf *p, const float *image, float 
             if (g->cylwiny[2 * x + 1] == 0) continue;
             for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++) {
                 float norm = 0;
-                float *o = buf + ((size_t)x * N + y) * Z4;
-                memset(acc, 0, sizeof(float) * (size_t)Z4);
+                og_real *o = buf + ((size_t)x * N + y) * Z4;
+                memset(acc, 0, sizeof(og_real) * (size_t)Z4);
                 for (mask = -h; mask < h; mask++) {
                     const int m2 = x + mask;
-                    const float *src;
+                    const og_real *src;
                     if (m2 < g->cylwiny[2 * y] || m2 >= g->cylwiny[2 * y + 1]) continue;
                     norm += f[mask + h];
                     src = image + ((size_t)m2 * N + y) * Z4;
@@ -124,7 +124,7 @@ void og_convolve3d(const og_geom *g, const og_psf *p, const float *image, float 
                     const float coef = 1 / norm;
                     for (z = 0; z < Z4; z++) o[z] = acc[z] * coef;
                 } else {
-                    memcpy(o, acc, sizeof(float) * (size_t)Z4);
+                    memcpy(o, acc, sizeof(og_real) * (size_t)Z4);
                 }
             }
         }
@@ -136,17 +136,17 @@ void og_convolve3d(const og_geom *g, const og_psf *p, const float *image, float 
             if (g->cylwiny[2 * x + 1] == 0) continue;
             for (y = g->cylwiny[2 * x]; y < g->cylwiny[2 * x + 1]; y++) {
                 float norm = 0;
-                float *o = out + ((size_t)x * N + y) * Z4;
-                memset(acc, 0, sizeof(float) * (size_t)Z4);
+                og_real *o = out + ((size_t)x * N + y) * Z4;
+                memset(acc, 0, sizeof(og_real) * (size_t)Z4);
                 for (mask = -h; mask < h; mask++) {
                     const int m2 = y + mask;
-                    const float *src;
+                    const og_real *src;
                     if (m2 < g->cylwiny[2 * x] || m2 >= g->cylwiny[2 * x + 1]) continue;
                     norm += f[mask + h];
                     src = buf + ((size_t)x * N + m2) * Z4;
                     for (z = 0; z < Z4; z++) acc[z] = acc[z] + src[z] * f[mask + h];
                 }
-                memset(o, 0, sizeof(float) * (size_t)Z4);
+                memset(o, 0, sizeof(og_real) * (size_t)Z4);
                 if (norm != 0) {
                     float coef;
                     for (mask = -h; mask < h; mask++) {

@@ -1,0 +1,215 @@
+"""Parity of the exact path the benchmark times, at its own shapes.
+
+- Whole subsets at span 3, 256x256 (9 view pairs per subset, fused ratio epilogue,
+  incremental back projection, rotation + update), bit-exact against the oracle
+  under the reference's thread schedules -T 1 (every pair incremental, the default)
+  and -T 8 (8 pairs incremental, 1 direct: hrrt_osem3d_span3.cpp:4174-4218).
+- A full iteration at span 9, 128x128 (16 subsets in sequence order, then the
+  end-of-iteration threshold, hrrt_osem3d_span3.cpp:4148-4297).
+- Every launch shape the autotuner may pick (forward tasks per thread and
+  workgroups per ray, back-projection waves per workgroup): the shapes must not
+  change a single bit.
+- Batched frames after the per-frame scratch grows: graphs captured before the
+  growth must not be replayed on freed buffers.
+"""
+import numpy as np
+import pytest
+
+from conftest import GEOMS
+from gpu_util import from_dev, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    torch.cuda.set_device(0)
+    return torch
+
+
+def _sinograms(torch, g, seed, lam=3.0, add_max=0.3):
+    """Seeded prompts (Poisson) and additive terms generated on the device in pair-column
+    layout; the host copies in the reference's [S][views][N] layout for the oracle."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    shape = (g.views // 2, 2, g.N, g.spp)
+    d_pr = torch.poisson(torch.full(shape, lam, device="cuda"), generator=gen).to(torch.int16)
+    d_add = torch.rand(shape, device="cuda", generator=gen) * add_max
+    # guard planes must stay zero (staging writes them as zero)
+    planes, pidx = g.paircol_index()
+    keep = torch.zeros(g.spp, dtype=torch.bool, device="cuda")
+    keep[torch.from_numpy(pidx).cuda()] = True
+    d_pr[..., ~keep] = 0
+    d_add[..., ~keep] = 0.0
+    torch.cuda.synchronize()
+    prompt = g.paircol_to_sino(d_pr.cpu().numpy())
+    add = g.paircol_to_sino(d_add.cpu().numpy())
+    return d_pr, d_add, prompt, add
+
+
+@pytest.fixture(scope="module")
+def c2_data(torch_dev, oracle_lib):
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    print("c2 data: generating sinograms", flush=True)
+    o = oracle_lib.Oracle(**GEOMS["c2"])
+    g = Osem3D(**GEOMS["c2"])
+    d_pr, d_add, prompt, add = _sinograms(torch, g, seed=71)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(72)
+    d_nf = torch.rand((o.subsets, o.N, o.N, o.Z4), device="cuda", generator=gen) * 1.95 + 0.05
+    print("c2 data: ready", flush=True)
+    return o, g, d_pr, d_add, prompt, add, d_nf
+
+
+@pytest.mark.parametrize("nthreads", [1, 8])
+def test_osem_subsets_c2_bitexact(torch_dev, c2_data, nthreads):
+    """The benchmark's subset update at its own shape, two subsets in sequence order."""
+    o, g, d_pr, d_add, prompt, add, d_nf = c2_data
+    g.set_backproject_schedule(nthreads)
+    img = o.init_image()
+    d_img = to_dev(img)
+    corr = o.image_zeros()
+    for s in [int(v) for v in o.order[:2]]:
+        print(f"c2 subset {s} nthreads={nthreads}", flush=True)
+        g.osem_subset(d_img, d_nf, d_pr, d_add, s)
+        nf = from_dev(d_nf[s])
+        o.osem_subset(img, corr, nf, add, prompt, s, nthreads=nthreads, workers=16)
+        out = from_dev(d_img)
+        assert np.array_equal(out, img), f"subset {s}: max |diff| {np.abs(out - img).max()} rel {rel_l2(out, img)}"
+    g.set_backproject_schedule(1)
+
+
+@pytest.mark.parametrize("nthreads", [1, 8])
+def test_osem_full_iteration_c1_bitexact(torch_dev, oracle_lib, nthreads):
+    """CalculateOsem3d's whole iteration at span 9, 128x128: 16 subsets in sequence order
+    and the end-of-iteration threshold.  With -T 8 and 4 pairs per subset every pair takes
+    the direct projector (the reference's remainder rule)."""
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    o = oracle_lib.Oracle(**GEOMS["c1"])
+    g = Osem3D(**GEOMS["c1"])
+    g.set_backproject_schedule(nthreads)
+    d_pr, d_add, prompt, add = _sinograms(torch, g, seed=73)
+    rng = np.random.default_rng(74)
+    nf = rng.uniform(0.05, 2.0, (o.subsets, o.N, o.N, o.Z4)).astype(np.float32)
+    d_nf = to_dev(nf)
+    img = o.init_image()
+    d_img = to_dev(img)
+    corr = o.image_zeros()
+    for i in range(o.subsets):
+        s = int(o.order[i])
+        print(f"c1 iteration subset {s} nthreads={nthreads}", flush=True)
+        o.osem_subset(img, corr, np.ascontiguousarray(nf[s]), add, prompt, s, nthreads=nthreads, workers=16)
+        g.osem_subset(d_img, d_nf, d_pr, d_add, s)
+    o.end_iteration(img)
+    g.end_iteration(d_img)
+    out = from_dev(d_img)
+    assert np.array_equal(out, img), f"max |diff| {np.abs(out - img).max()} rel {rel_l2(out, img)}"
+    assert (img > 0).sum() > 0
+
+
+@pytest.mark.parametrize("geom", ["tiny", "c1"])
+def test_autotuner_shapes_bitexact(torch_dev, oracle_lib, geom):
+    """Every forward shape and back-projection workgroup size of the autotuner's candidate
+    lists gives the oracle's subset update bit for bit (direct and incremental schedules)."""
+    from hrrt_reconstruction_amd import autotune
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    o = oracle_lib.Oracle(**GEOMS[geom])
+    g = Osem3D(**GEOMS[geom])
+    d_pr, d_add, prompt, add = _sinograms(torch, g, seed=75)
+    rng = np.random.default_rng(76)
+    s = 3
+    nf = rng.uniform(0.05, 2.0, (o.subsets, o.N, o.N, o.Z4)).astype(np.float32)
+    d_nf = to_dev(nf)
+    img0 = o.init_image() * rng.uniform(0.5, 2.0, (o.N, o.N, o.Z4)).astype(np.float32)
+    refs = {}
+    for T in (0, 1):
+        ref = img0.copy()
+        o.osem_subset(ref, o.image_zeros(), np.ascontiguousarray(nf[s]), add, prompt, s, nthreads=T, workers=16)
+        refs[T] = ref
+    for shape in autotune.CANDIDATES:
+        g.set_launch_shape(*shape)
+        for T in (0, 1):
+            print(f"autotune {geom} forward shape {shape} T={T}", flush=True)
+            g.set_backproject_schedule(T)
+            d_img = to_dev(img0)
+            g.osem_subset(d_img, d_nf, d_pr, d_add, s)
+            assert np.array_equal(from_dev(d_img), refs[T]), f"forward shape {shape}, T={T}"
+    g.set_launch_shape(4, 0)
+    g.set_backproject_schedule(0)
+    for w in autotune.BP_CANDIDATES:
+        print(f"autotune {geom} back-projection waves {w}", flush=True)
+        g.set_backproject_shape(w)
+        d_img = to_dev(img0)
+        g.osem_subset(d_img, d_nf, d_pr, d_add, s)
+        assert np.array_equal(from_dev(d_img), refs[0]), f"back-projection waves {w}"
+
+
+def test_autotuner_keeps_the_schedule(torch_dev, oracle_lib):
+    """The tuner times shapes only: the caller's back-projection schedule (the reference's
+    -T) is never changed by tuning."""
+    from hrrt_reconstruction_amd import autotune
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    g = Osem3D(**GEOMS["tiny"])
+    o = oracle_lib.Oracle(**GEOMS["tiny"])
+    d_pr, d_add, _, _ = _sinograms(torch, g, seed=77)
+    d_nf = to_dev(np.ones((o.subsets, o.N, o.N, o.Z4), np.float32))
+    img = to_dev(o.init_image())
+    for T in (0, 1, 3):
+        g.set_backproject_schedule(T)
+        autotune.tune_backproject(g, img, d_nf, d_pr, d_add, repeats=1)
+        assert g.backproject_schedule == T
+        autotune.tune_forward(g, img, d_nf, d_pr, d_add, repeats=1)
+        assert g.backproject_schedule == T
+
+
+def test_frames_graphs_survive_scratch_growth(torch_dev, oracle_lib):
+    """iteration_frames(2) captures graphs; subset_frames(3) grows the per-frame scratch
+    (freeing the old buffers); iteration_frames(2) again must recapture, not replay graphs
+    that point at the freed buffers.  Every result equals eager single-frame updates."""
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    kw = dict(GEOMS["tiny"], subsets=4)
+    o = oracle_lib.Oracle(**kw)
+    g = Osem3D(**kw)
+    rng = np.random.default_rng(78)
+    shape = (o.S, o.views, o.N)
+    prompts = [rng.poisson(1.0 + 2 * f, shape).astype(np.int16) for f in range(3)]
+    adds = [rng.uniform(0.0, 0.3, shape).astype(np.float32) for _ in range(3)]
+    nf = to_dev(rng.uniform(0.05, 2.0, (o.subsets, o.N, o.N, o.Z4)).astype(np.float32))
+    d_pr = to_dev(np.stack([g.sino_to_paircol(p) for p in prompts]))
+    d_add = to_dev(np.stack([g.sino_to_paircol(a) for a in adds]))
+    img0 = o.init_image()
+    imgs = to_dev(np.stack([img0] * 3))
+    eager = [to_dev(img0) for _ in range(3)]
+    stream = torch.cuda.Stream()
+
+    def check(step):
+        got = from_dev(imgs)
+        for f in range(3):
+            assert np.array_equal(got[f], from_dev(eager[f])), f"{step}: frame {f}"
+
+    print("frames scratch growth: iteration of 2 frames", flush=True)
+    with torch.cuda.stream(stream):
+        g.osem_iteration_frames(imgs[:2], nf, d_pr[:2], d_add[:2])
+    stream.synchronize()
+    for f in range(2):
+        g.calculate_osem3d(eager[f], nf, d_pr[f], d_add[f], iterations=1)
+    check("first iteration")
+    print("frames scratch growth: one subset of 3 frames", flush=True)
+    g.osem_subset_frames(imgs, nf, d_pr, d_add, 1)
+    for f in range(3):
+        g.osem_subset(eager[f], nf, d_pr[f], d_add[f], 1)
+    check("subset of 3 frames")
+    print("frames scratch growth: iteration of 2 frames again", flush=True)
+    with torch.cuda.stream(stream):
+        g.osem_iteration_frames(imgs[:2], nf, d_pr[:2], d_add[:2])
+    stream.synchronize()
+    for f in range(2):
+        g.calculate_osem3d(eager[f], nf, d_pr[f], d_add[f], iterations=1)
+    check("second iteration")

@@ -1,0 +1,156 @@
+"""The precision form (FP16 per-bin factors, FP32 arithmetic, compensated sums) and the
+batched dynamic form measured against the FP64 oracle, at the workloads they are quoted on.
+
+The FP64 oracle (liboracle64.so) applies the reference's own discretisation -- its float
+rotation positions, axial positions and scale factors -- with double arithmetic, so the
+differences below are the GPU forms' arithmetic and storage errors alone.  Its sensitivity
+and subsets take the incremental back projector for every pair (in double the two back
+projectors agree to ~1e-8).  Tolerance of every comparison: relative L2 <= 2e-4.
+
+- Precision form at span 9, 128x128x207 (the reference's -X 128 configuration): the
+  forward sinogram, the sensitivity images of every subset (FP16 1/(N*A), Kahan sums), and
+  the image after 2 iterations of 16 subsets (FP16 additive term).
+- Batched dynamic form at span 3, 256x256x207: 2 frames of different counts with randoms
+  and scatter, FP16 factors, one graph-replayed iteration for both frames; each frame's
+  image against the FP64 oracle's.
+"""
+import numpy as np
+import pytest
+
+from conftest import GEOMS
+from gpu_util import from_dev, rel_l2, to_dev
+
+pytestmark = pytest.mark.gpu
+
+TOL = 2e-4
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible GPU"
+    torch.cuda.set_device(0)
+    return torch
+
+
+def _study(torch, g, counts, seed, with_scatter, factors=None):
+    """Phantom, staged prompts, additive term and 1/(N*A); factors = (norm, atten) of an
+    earlier frame of the same study, or None for new ones (returned for the next frame)."""
+    from hrrt_reconstruction_amd import phantom as ph
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    cg = torch.Generator()
+    cg.manual_seed(seed)
+    norm, atten = factors if factors else (None, None)
+    img, st = ph.make_geometry_study(g, counts, gen, cg, with_scatter=with_scatter, norm=norm, atten=atten)
+    pr, ad, inv = g.stage(st["prompt"], st["delayed"], st["norm"], st["scatter"], st["atten"])
+    torch.cuda.synchronize()
+    return img, pr, ad, inv, (st["norm"], st["atten"])
+
+
+@pytest.fixture(scope="module")
+def c1_study(torch_dev, oracle_lib):
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    print("c1 study: phantom, 50M prompts, randoms 10%", flush=True)
+    kw = GEOMS["c1"]
+    g = Osem3D(**kw)
+    o64 = oracle_lib.Oracle(double=True, **kw)
+    img, pr, ad, inv, _ = _study(torch, g, 50e6, 91, with_scatter=False)
+    o64.set_norm_zero_mask(np.pad(g.norm_zero_mask(), ((0, 1), (0, 0))))
+    host = dict(prompt=g.paircol_to_sino(from_dev(pr)), add=g.paircol_to_sino(from_dev(ad)).astype(np.float64),
+                inv=g.paircol_to_sino(from_dev(inv)).astype(np.float64))
+    print("c1 study: ready", flush=True)
+    return g, o64, img, pr, ad, inv, host
+
+
+def test_c3_forward_vs_fp64(torch_dev, c1_study):
+    g, o64, img, pr, ad, inv, host = c1_study
+    img64 = from_dev(img).astype(np.float64)
+    for v in (0, 37, 71):
+        e = rel_l2(from_dev(g.forward_proj3d(img, v)), o64.forward(img64, v))
+        print(f"c3 forward view {v}: rel-L2 vs FP64 {e:.2e}", flush=True)
+        assert e < TOL
+
+
+def test_c3_sensitivity_vs_fp64(torch_dev, c1_study):
+    g, o64, img, pr, ad, inv, host = c1_study
+    print("c3 sensitivity: FP64 oracle ...", flush=True)
+    ref = o64.calculate_normfac(host["inv"].copy(), nthreads=g.sviews // 2)
+    nf16 = from_dev(g.calculate_normfac_mixed(g.to_half(inv)))
+    nf32 = from_dev(g.calculate_normfac(inv))
+    worst16 = max(rel_l2(nf16[s], ref[s]) for s in range(g.subsets))
+    worst32 = max(rel_l2(nf32[s], ref[s]) for s in range(g.subsets))
+    print(f"c3 sensitivity: worst subset rel-L2 vs FP64: fp16+kahan {worst16:.2e}, fp32 {worst32:.2e}", flush=True)
+    assert worst16 < TOL and worst32 < TOL
+
+
+def test_c3_image_after_two_iterations_vs_fp64(torch_dev, c1_study):
+    g, o64, img, pr, ad, inv, host = c1_study
+    print("c3 image: FP64 oracle sensitivity ...", flush=True)
+    nf64 = o64.calculate_normfac(host["inv"].copy(), nthreads=g.sviews // 2)
+    inv16 = g.to_half(inv)
+    nf16 = g.calculate_normfac_mixed(inv16)
+    nf32 = g.calculate_normfac(inv)
+    ad16 = g.to_half(ad)
+    i16, i32 = g.init_image(), g.init_image()
+    ref = o64.init_image()
+    corr = o64.image_zeros()
+    for it in range(2):
+        for i in range(g.subsets):
+            s = int(g.order[i])
+            g.osem_subset_mixed(i16, nf16, pr, ad16, s)
+            g.osem_subset(i32, nf32, pr, ad, s)
+            o64.osem_subset(ref, corr, np.ascontiguousarray(nf64[s]), host["add"], host["prompt"], s, nthreads=1,
+                            workers=16)
+        g.end_iteration(i16)
+        g.end_iteration(i32)
+        o64.end_iteration(ref)
+        print(f"c3 image: iteration {it + 1} done", flush=True)
+    e16, e32 = rel_l2(from_dev(i16), ref), rel_l2(from_dev(i32), ref)
+    print(f"c3 image after 2 iterations: rel-L2 vs FP64: fp16 form {e16:.2e}, fp32 {e32:.2e}", flush=True)
+    assert e16 < TOL and e32 < TOL
+
+
+def test_c5_batched_frames_vs_fp64(torch_dev, oracle_lib):
+    """Two span-3 frames (1M-class and 30M-class counts, randoms 10%, scatter 15%) in the
+    batched FP16 form: one graph-replayed iteration, each frame against the FP64 oracle."""
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    kw = GEOMS["c2"]
+    g = Osem3D(**kw)
+    o64 = oracle_lib.Oracle(double=True, **kw)
+    frames, factors = [], None
+    for f, counts in enumerate((1.5e6, 30e6)):
+        print(f"c5 frame {f}: {counts / 1e6:.1f}M prompts", flush=True)
+        fr = _study(torch, g, counts, 95 + f, with_scatter=True, factors=factors)
+        factors = fr[4]                       # one norm and attenuation for the study
+        frames.append(fr[:4])
+    inv = frames[0][3]
+    o64.set_norm_zero_mask(np.pad(g.norm_zero_mask(), ((0, 1), (0, 0))))
+    inv16 = g.to_half(inv)
+    nf16 = g.calculate_normfac_mixed(inv16)
+    prompts = torch.stack([fr[1] for fr in frames])
+    adds16 = g.to_half(torch.stack([fr[2] for fr in frames]))
+    imgs = torch.stack([g.init_image() for _ in frames])
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        g.osem_iteration_frames_mixed(imgs, nf16, prompts, adds16)
+    stream.synchronize()
+    got = from_dev(imgs)
+    print("c5: FP64 oracle sensitivity ...", flush=True)
+    inv64 = g.paircol_to_sino(from_dev(inv)).astype(np.float64)
+    nf64 = o64.calculate_normfac(inv64, nthreads=g.sviews // 2)
+    del inv64
+    for f, fr in enumerate(frames):
+        add64 = g.paircol_to_sino(from_dev(fr[2])).astype(np.float64)
+        prompt = g.paircol_to_sino(from_dev(fr[1]))
+        ref = o64.init_image()
+        corr = o64.image_zeros()
+        for i in range(g.subsets):
+            s = int(g.order[i])
+            o64.osem_subset(ref, corr, np.ascontiguousarray(nf64[s]), add64, prompt, s, nthreads=1, workers=16)
+        o64.end_iteration(ref)
+        e = rel_l2(got[f], ref)
+        print(f"c5 frame {f}: image after 1 iteration rel-L2 vs FP64 {e:.2e}", flush=True)
+        assert e < TOL
