@@ -313,39 +313,36 @@ def batched_frames(torch, nframes, iterations, seed):
     nf16 = g.calculate_normfac_mixed(inv16)
     adds16 = g.to_half(adds)
     stream = torch.cuda.Stream()
-    res = {}
+    res, out = {}, {}
+    init = g.init_image()
     for form in ("fp32", "fp16"):
-        imgs = torch.stack([g.init_image() for _ in range(nframes)])
+        imgs = torch.stack([init for _ in range(nframes)])
         ll = torch.zeros((g.subsets, nframes), dtype=torch.float64, device="cuda")
+
+        def iterate(n):
+            stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
+            with torch.cuda.stream(stream):
+                for _ in range(n):
+                    if form == "fp32":
+                        g.osem_iteration_frames(imgs, nf32, prompts, adds)
+                    else:
+                        g.osem_iteration_frames_mixed(imgs, nf16, prompts, adds16, invnorm16=inv16, loglik=ll)
+            stream.synchronize()
+        iterate(1)                            # captures the subsets' graphs
+        imgs.copy_(init.expand_as(imgs))      # same buffers: the graphs are replayed
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        with torch.cuda.stream(stream):
-            for _ in range(iterations):
-                if form == "fp32":
-                    g.osem_iteration_frames(imgs, nf32, prompts, adds)
-                else:
-                    g.osem_iteration_frames_mixed(imgs, nf16, prompts, adds16, invnorm16=inv16, loglik=ll)
-        stream.synchronize()
-        dt = time.perf_counter() - t0
-        res[form] = (imgs, dt, ll)
-        log(f"batched {form}: {nframes} frames x {g.subsets} subsets x {iterations} iterations in {dt:.3f} s "
-            f"(graphs captured on first use)")
-    # time again with the graphs already instantiated
-    out = {}
-    for form in ("fp32", "fp16"):
-        imgs = torch.stack([g.init_image() for _ in range(nframes)])
-        ll = res[form][2]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        with torch.cuda.stream(stream):
-            for _ in range(iterations):
-                if form == "fp32":
-                    g.osem_iteration_frames(imgs, nf32, prompts, adds)
-                else:
-                    g.osem_iteration_frames_mixed(imgs, nf16, prompts, adds16, invnorm16=inv16, loglik=ll)
-        stream.synchronize()
+        iterate(iterations)
         out[form] = time.perf_counter() - t0
         res[form] = (imgs, out[form], ll)
+        log(f"batched {form}: {nframes} frames x {g.subsets} subsets x {iterations} iterations in {out[form]:.3f} s")
+    # the single-frame FP32 path on one frame, for the per-frame rate without batching
+    img = init.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.calculate_osem3d(img, nf32, prompts[0], adds[0], iterations=iterations)
+    torch.cuda.synchronize()
+    out["single"] = time.perf_counter() - t0
     a, b = res["fp32"][0], res["fp16"][0]
     rel = [float(torch.linalg.vector_norm((b[f] - a[f]).double()) / torch.linalg.vector_norm(a[f].double()))
            for f in range(nframes)]
@@ -358,9 +355,147 @@ def batched_frames(torch, nframes, iterations, seed):
         "fp32_frames_per_s": round(nframes / out["fp32"], 3),
         "fp16_frames_per_s": round(nframes / out["fp16"], 3),
         "fp16_vs_fp32": round(out["fp32"] / out["fp16"], 3),
+        "fp32_single_frame_frames_per_s": round(1.0 / out["single"], 3),
+        "fp16_batched_vs_fp32_single": round(out["single"] * nframes / out["fp16"], 3),
         "fp16_image_rel_l2_vs_fp32_max": float(f"{max(rel):.3e}"),
         "fp16_loglik_last_subset": [float(x) for x in res["fp16"][2][-1].cpu().numpy()[:3]],
     }
+
+
+def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
+    """Combined form at full scale: a span-3 256x256x207 dynamic study of nframes frames (counts
+    log-uniform between 1M and 300M, spheres washing out, randoms 10 %, scatter 15 %), FP16
+    additive terms and sensitivity weights, int16 prompts, FP32 images; every subset of an
+    iteration replayed for all frames from a HIP graph.  Also timed: the FP32 batched form on
+    fp32_frames frames and the FP32 single-frame path on one frame (frames/s of each), and the
+    image of fp64_frames frames (lowest and highest counts first) against the FP64 oracle."""
+    import numpy as np
+    from hrrt_reconstruction_amd import phantom as ph
+    from hrrt_reconstruction_amd.osem import Osem3D
+    g = Osem3D("cuda:0", **GEOM)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    cg = torch.Generator()
+    cg.manual_seed(seed)
+    dur, mids, counts, rates = ph.dynamic_frames(nframes, 1e6, 300e6, cg)
+    pc = (g.views // 2, 2, g.N, g.spp)
+    prompts = torch.zeros((nframes,) + pc, dtype=torch.int16, device="cuda")
+    adds16 = torch.zeros((nframes,) + pc, dtype=torch.int16, device="cuda")
+    keep = sorted(range(nframes), key=lambda f: counts[f])
+    keep = ([keep[0], keep[-1]] + keep[1:-1])[:max(fp64_frames, fp32_frames, 1)]
+    adds32 = {}
+    factors, invnorm = None, None
+    for f in range(nframes):
+        act = [1.0 + 3.0 * math.exp(-r * mids[f]) for r in rates]
+        pg = torch.Generator()
+        pg.manual_seed(seed + 1)                 # same sphere positions in every frame
+        _, st = ph.make_geometry_study(g, counts[f], gen, pg, with_scatter=True, sphere_activity=act,
+                                       norm=factors[0] if factors else None, atten=factors[1] if factors else None)
+        factors = (st["norm"], st["atten"])
+        pr, ad, inv = g.stage(st["prompt"], st["delayed"], st["norm"], st["scatter"], st["atten"])
+        prompts[f].copy_(pr)
+        adds16[f].copy_(g.to_half(ad))
+        if f in keep:
+            adds32[f] = ad
+        if invnorm is None:
+            invnorm = inv
+        del st, pr, inv
+        if f % 8 == 7:
+            torch.cuda.synchronize()
+            log(f"c5: {f + 1}/{nframes} frames generated and staged")
+    del factors
+    torch.cuda.synchronize()
+    inv16 = g.to_half(invnorm)
+    nf16 = g.calculate_normfac_mixed(inv16)
+    nf32 = g.calculate_normfac(invnorm)
+    stream = torch.cuda.Stream()
+    out = {}
+
+    def run_batched(imgs, fp16, pr, ad, iters):
+        stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
+        with torch.cuda.stream(stream):
+            for _ in range(iters):
+                if fp16:
+                    g.osem_iteration_frames_mixed(imgs, nf16, pr, ad)
+                else:
+                    g.osem_iteration_frames(imgs, nf32, pr, ad)
+        stream.synchronize()
+
+    # FP16 batched form, all frames: once to capture the graphs, then timed on the same buffers
+    init = g.init_image()
+    imgs = torch.stack([init for _ in range(nframes)])
+    run_batched(imgs, True, prompts, adds16, 1)
+    imgs.copy_(init.expand_as(imgs))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_batched(imgs, True, prompts, adds16, iterations)
+    t16 = time.perf_counter() - t0
+    log(f"c5: fp16 batched {nframes} frames x {g.subsets} subsets x {iterations} iterations in {t16:.3f} s")
+    out["fp16_batched_frames_per_s"] = round(nframes / t16, 3)
+    # FP32 batched form on fp32_frames frames (the same work per frame)
+    sel = keep[:fp32_frames]
+    if sel:
+        pr32 = torch.stack([prompts[f] for f in sel])
+        ad32 = torch.stack([adds32[f] for f in sel])
+        i32 = torch.stack([init for _ in sel])
+        run_batched(i32, False, pr32, ad32, 1)
+        i32.copy_(init.expand_as(i32))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run_batched(i32, False, pr32, ad32, iterations)
+        t32 = time.perf_counter() - t0
+        out["fp32_batched_frames_per_s"] = round(len(sel) / t32, 3)
+        out["fp32_batched_frames"] = len(sel)
+        # FP32 single-frame path, one frame after another (the plain GPU form)
+        img = g.init_image()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.calculate_osem3d(img, nf32, prompts[sel[0]], adds32[sel[0]], iterations=iterations)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter() - t0
+        out["fp32_single_frame_frames_per_s"] = round(1.0 / t1, 3)
+        out["fp16_batched_vs_fp32_single"] = round(out["fp16_batched_frames_per_s"] / out["fp32_single_frame_frames_per_s"], 3)
+        out["fp16_batched_vs_fp32_batched"] = round(out["fp16_batched_frames_per_s"] / out["fp32_batched_frames_per_s"], 3)
+        del pr32, ad32, i32
+    # FP64 oracle images of the first fp64_frames kept frames
+    if fp64_frames > 0:
+        from oracle import oracle
+        oracle.build()
+        o64 = oracle.Oracle(double=True, **GEOM)
+        o64.set_norm_zero_mask(np.pad(g.norm_zero_mask(), ((0, 1), (0, 0))))
+        inv64 = paircol_to_sino(g, torch, invnorm).cpu().numpy().astype(np.float64)
+        ncores = min(len(os.sched_getaffinity(0)), 16)
+        nf64 = o64.calculate_normfac(inv64, nthreads=g.sviews // 2)
+        del inv64
+        got = imgs.cpu().numpy()
+        rel = {}
+        for f in keep[:fp64_frames]:
+            add64 = paircol_to_sino(g, torch, adds32[f]).cpu().numpy().astype(np.float64)
+            pr = paircol_to_sino(g, torch, prompts[f]).cpu().numpy()
+            ref = o64.init_image()
+            corr = o64.image_zeros()
+            for _ in range(iterations):
+                for i in range(g.subsets):
+                    s = int(g.order[i])
+                    o64.osem_subset(ref, corr, np.ascontiguousarray(nf64[s]), add64, pr, s, nthreads=1, workers=ncores)
+                o64.end_iteration(ref)
+            d = got[f].astype(np.float64) - ref
+            rel[f] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+            log(f"c5: frame {f} ({counts[f] / 1e6:.1f}M prompts) image rel-L2 vs FP64 oracle {rel[f]:.2e}")
+        out["fp16_image_rel_l2_vs_fp64"] = {str(f): float(f"{v:.3e}") for f, v in rel.items()}
+        out["fp64_reference"] = f"FP64 oracle, {iterations} iterations, {ncores} host threads"
+    out.update({
+        "workload": f"dynamic study, HRRT span 3, 256x256x207 image, {nframes} frames (log-uniform 1M-300M prompts, "
+                    "spheres washing out, randoms 10%, scatter 15%), FP16 additive terms and sensitivity weights, "
+                    f"int16 prompts, FP32 images, {g.subsets} subsets x {iterations} iterations, subsets replayed "
+                    "from HIP graphs for all frames",
+        "frames": nframes, "iterations": iterations,
+        "counts_m": [round(c / 1e6, 2) for c in counts],
+    })
+    del prompts, adds16, adds32, imgs, nf16, nf32
+    g.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -383,6 +518,13 @@ def main():
     ap.add_argument("--listmode-events", type=int, default=40_000_000,
                     help="events of the synthetic list-mode stream histogrammed per step (0 = skip)")
     ap.add_argument("--listmode-steps", type=int, default=5)
+    ap.add_argument("--c5-frames", type=int, default=48,
+                    help="frames of the span-3 FP16 batched dynamic study (0 = skip)")
+    ap.add_argument("--c5-iterations", type=int, default=3)
+    ap.add_argument("--c5-fp32-frames", type=int, default=4,
+                    help="frames of that study also run in the FP32 batched and single-frame forms")
+    ap.add_argument("--c5-fp64-frames", type=int, default=2,
+                    help="frames of that study checked against the FP64 oracle on the host (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -490,6 +632,14 @@ def main():
     batched = None
     if args.frames > 0:
         batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
+    schedule = g.backproject_schedule
+    c5 = None
+    if args.c5_frames > 0:
+        del prompt, add, normfac, image, image0
+        g.close()
+        torch.cuda.empty_cache()
+        c5 = c5_leg(torch, args.c5_frames, args.c5_iterations, args.seed + 17,
+                    args.c5_fp64_frames if args.cpu_subsets > 0 else 0, args.c5_fp32_frames)
 
     out = {
         "metric": "OSEM subset-iters/s span3 256²×207; batched frames/s; % HBM roofline",
@@ -509,9 +659,8 @@ def main():
                    "subsets": g.subsets, "view_pairs_per_subset": npairs,
                    "stage_s": round(stage_s, 3), "normfac_s": round(normfac_s, 3),
                    "fp_launch_shape": list(shape) if shape else None, "direct_bp_waves": bp_choice,
-                   "back_projector": "incremental" if g.backproject_schedule == 1 else
-                   ("direct" if g.backproject_schedule == 0 else "mixed"),
-                   "backproject_schedule": g.backproject_schedule},
+                   "back_projector": "incremental" if schedule == 1 else ("direct" if schedule == 0 else "mixed"),
+                   "backproject_schedule": schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic},
@@ -528,6 +677,7 @@ def main():
         "randoms": randoms,
         "listmode": listmode,
         "batched": batched,
+        "c5": c5,
     }
     print(json.dumps(out), flush=True)
 

@@ -290,7 +290,9 @@ class Osem3D(Layout):
                   self._pc(adds, torch.float32, "adds", n), int(isubset), _stream())
 
     def osem_iteration_frames(self, images, normfac, prompts, adds):
-        """One iteration (all subsets, graph-replayed, then the end-of-iteration threshold) for every frame."""
+        """One iteration (all subsets, graph-replayed, then the end-of-iteration threshold) for every frame,
+        on the current stream, which must not be the default one.  As for any stream, work that writes
+        the inputs on another stream must be ordered before it (torch: stream.wait_stream(...))."""
         s = torch.cuda.current_stream()
         if s.cuda_stream == 0:
             raise ValueError("graph replay needs a non-default stream (use torch.cuda.stream(...))")

@@ -193,6 +193,9 @@ void     og_image_from_flat(const og_geom *g, const float *flat, float *image, i
  * nthreads selects the reference's split between the *_thread1 and *2 variants
  * (S3:3796-3825); nthreads<=0 means "direct variant for every pair". */
 void     og_calculate_normfac(const og_geom *g, og_real *invnorm, og_real *normfac_all, int nthreads);
+/* The same back projection of invnorm per subset without the inversion (the sensitivity
+ * images themselves): sens_all [subsets][X][Y][Z4]. */
+void     og_calculate_sensitivity(const og_geom *g, og_real *invnorm, og_real *sens_all, int nthreads);
 /* One OSEM subset update (CalculateOsem3d body S3:4165-4279) on subset isubset.
  * nthreads as above; workers>1 runs the view pairs on that many POSIX threads
  * (the arithmetic does not depend on workers). */

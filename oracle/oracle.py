@@ -97,6 +97,7 @@ def _bind(L, real_ptr_is_double):
     L.og_gather_pair_f.argtypes = [G, vp, C.c_int, vp]
     L.og_gather_pair_s.argtypes = [G, vp, C.c_int, vp]
     L.og_calculate_normfac.argtypes = [G, vp, vp, C.c_int]
+    L.og_calculate_sensitivity.argtypes = [G, vp, vp, C.c_int]
     L.og_osem_subset.argtypes = [G, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
     L.og_osem_subset_w012.argtypes = [G, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int]
     L.og_psf_create.restype = vp
@@ -408,6 +409,12 @@ class Oracle:
         nf = np.zeros((self.subsets, self.N, self.N, self.Z4), self.real)
         self.L.og_calculate_normfac(self.gp, self._v(invnorm), self._v(nf), nthreads)
         return nf
+
+    def calculate_sensitivity(self, invnorm, nthreads=0):
+        """The sensitivity images before inversion, [subsets][X][Y][Z4] (invnorm is rescaled in place)."""
+        sens = np.zeros((self.subsets, self.N, self.N, self.Z4), self.real)
+        self.L.og_calculate_sensitivity(self.gp, self._v(invnorm), self._v(sens), nthreads)
+        return sens
 
     def osem_subset(self, image, correction, normfac_subset, add, prompt, isubset, nthreads=0, workers=1):
         self.L.og_osem_subset(self.gp, self._v(image), self._v(correction), self._v(normfac_subset),

@@ -537,6 +537,17 @@ void og_calculate_normfac(const og_geom *g, og_real *invnorm, og_real *normfac_a
     free(sens);
 }
 
+void og_calculate_sensitivity(const og_geom *g, og_real *invnorm, og_real *sens_all, int nthreads)
+{   /* CalculateNormfac S3:3787-3830 before the inversion: the back-projected weights per subset */
+    const size_t V = (size_t)g->N * g->N * g->z4;
+    int s;
+    for (s = 0; s < g->p.subsets; s++) {
+        memset(sens_all + (size_t)s * V, 0, sizeof(og_real) * V);
+        run_subset_pairs(g, s, nthreads, nthreads > 0 ? nthreads : 1, 1, invnorm, NULL, NULL, NULL,
+                         sens_all + (size_t)s * V);
+    }
+}
+
 void og_osem_subset(const og_geom *g, og_real *image, og_real *correction, const og_real *normfac_subset,
                     const og_real *add, const int16_t *prompt, int isubset, int nthreads, int workers)
 {   /* CalculateOsem3d S3:4174-4245 */

@@ -195,6 +195,7 @@ def test_frames_graphs_survive_scratch_growth(torch_dev, oracle_lib):
             assert np.array_equal(got[f], from_dev(eager[f])), f"{step}: frame {f}"
 
     print("frames scratch growth: iteration of 2 frames", flush=True)
+    stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
     with torch.cuda.stream(stream):
         g.osem_iteration_frames(imgs[:2], nf, d_pr[:2], d_add[:2])
     stream.synchronize()
@@ -207,6 +208,7 @@ def test_frames_graphs_survive_scratch_growth(torch_dev, oracle_lib):
         g.osem_subset(eager[f], nf, d_pr[f], d_add[f], 1)
     check("subset of 3 frames")
     print("frames scratch growth: iteration of 2 frames again", flush=True)
+    stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
     with torch.cuda.stream(stream):
         g.osem_iteration_frames(imgs[:2], nf, d_pr[:2], d_add[:2])
     stream.synchronize()

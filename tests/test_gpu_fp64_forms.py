@@ -134,6 +134,7 @@ def test_c5_batched_frames_vs_fp64(torch_dev, oracle_lib):
     adds16 = g.to_half(torch.stack([fr[2] for fr in frames]))
     imgs = torch.stack([g.init_image() for _ in frames])
     stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
     with torch.cuda.stream(stream):
         g.osem_iteration_frames_mixed(imgs, nf16, prompts, adds16)
     stream.synchronize()

@@ -78,6 +78,7 @@ def test_frames_iteration_graph_replays(torch_dev, oracle_lib):
     eager = [to_dev(img0) for _ in range(nframes)]
     stream = torch.cuda.Stream()
     for it in range(3):
+        stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
         with torch.cuda.stream(stream):
             g.osem_iteration_frames(d_imgs, d_nf, d_pr, d_add)
         stream.synchronize()

@@ -198,6 +198,7 @@ def test_frames_mixed_match_single(torch_dev, tiny):
     ll_it = torch.zeros((o.subsets, F), dtype=torch.float64, device="cuda")
     prev = None
     for it in range(2):
+        stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
         with torch.cuda.stream(stream):
             g.osem_iteration_frames_mixed(d_imgs, nf, d_pr, d_add16, invnorm16=d_inv16, loglik=ll_it)
         stream.synchronize()

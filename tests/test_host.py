@@ -129,6 +129,20 @@ def test_invalid_geometry_is_rejected_without_device(bad, lib):
         lib.call("hrrt_query_geometry", C.byref(p), C.byref(d), *([None] * 6))
 
 
+@pytest.mark.parametrize("geom", ["tiny", "tinyrb", "tiny9", "c1", "c2"])
+def test_context_tables_pass_their_host_checks(geom, lib):
+    """hrrt_create builds and checks every table on the host (forward positions against the
+    lookup tables, the row-skip threshold, the forward projector's paired task list) before
+    the first device allocation: without a device it must get as far as that allocation."""
+    from hrrt_reconstruction_amd.osem import make_params
+    h = C.c_void_p()
+    rc = lib.lib().hrrt_create(C.byref(make_params(**GEOMS[geom])), C.byref(h))
+    if rc == lib.HRRT_OK:          # a device is present
+        lib.lib().hrrt_destroy(h)
+    else:
+        assert rc == lib.HRRT_ERR_NOMEM, lib.lib().hrrt_last_error()
+
+
 def test_null_arguments(lib):
     L = lib.lib()
     assert L.hrrt_create(None, None) == lib.HRRT_ERR_ARG
