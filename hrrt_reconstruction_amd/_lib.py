@@ -90,6 +90,7 @@ SIGNATURES = {
     "hrrt_calculate_normfac": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_init_image": (_I, [_VP, _VP, _VP]),
     "hrrt_osem_subset": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "hrrt_forward_proj3d_noalloc": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_stage_sinograms_w012": (_I, [_VP, _I, _I] + [_VP] * 8 + [_VP]),
     "hrrt_osem_subset_w012": (_I, [_VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_end_iteration": (_I, [_VP, _VP, _VP]),

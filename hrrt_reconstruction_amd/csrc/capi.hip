@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -37,6 +38,35 @@ int fail(int code, const char *fmt, ...) {
 
 }  // namespace
 
+// Scratch of hrrt_forward_proj3d_noalloc: one pair-column ([2][N][spp] floats) in a
+// file-scope device array, so the entry point allocates nothing.  Every GPU holds its own
+// copy for the life of the process.  In host code `scratch` only names the symbol: its
+// device address comes from hipGetSymbolAddress, taken once per device when a context is
+// created on it (kept in g_scratch_ptr), and only that address is passed to kernels.
+static __device__ __align__(256) float scratch[HRRT_NOALLOC_SCRATCH_FLOATS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+// Device address of `scratch` on the current device (looked up once per device).
+hipError_t scratch_address(int dev, float **out) {
+    *out = nullptr;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev] && g_scratch_err[dev] == hipSuccess) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) g_scratch_err[dev] = e;
+        else g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return g_scratch_ptr[dev] ? hipSuccess : g_scratch_err[dev];
+}
+}  // namespace
+
 struct hrrt_ctx {
     hrrt::Geometry geo;
     hrrt::DevGeom dg{};
@@ -50,6 +80,8 @@ struct hrrt_ctx {
     float *h_img = nullptr, *h_prj = nullptr, *h_buf = nullptr;  // device staging for the host shims
     float *rot_zero = nullptr;       // a zero rotated-frame half, for rotating a caller's buffer
     int device = 0;
+    float *noalloc_scratch = nullptr;   // this device's address of `scratch` (NULL: lookup failed)
+    hipError_t noalloc_err = hipSuccess;
     int bp_threads = 0;              // back-projection schedule (the reference's -T): 0 = direct for every pair
     // PSF resolution model (-B): host kernels (x, y, z), normz, their device copies, and three
     // scratch volumes (the blurred image, the correction, the x-pass buffer), zeroed once so
@@ -220,6 +252,8 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         return fail(HRRT_ERR_ARG, "%s", err.c_str());
     }
     (void)hipGetDevice(&ctx->device);
+    // the no-alloc entry point's scratch on this device (its failure is reported by that call)
+    ctx->noalloc_err = scratch_address(ctx->device, &ctx->noalloc_scratch);
     const int N = G.N, c = N / 2, GM = G.groupmax, ZS = G.zshift, Z = G.z_pixels;
     hrrt::DevGeom &d = ctx->dg;
     d.N = N; d.c = c; d.views = G.views; d.Z = Z; d.Z4 = G.z4; d.NQ = G.z4 / 4; d.S = G.segmentsize;
@@ -1035,6 +1069,21 @@ extern "C" int hrrt_forward_proj3d(hrrt_ctx *ctx, const float *d_image, float *d
     hrrt::PairBatch b{view, 0, 1};
     HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->pc1, hrrt::FpIO(), hrrt::FP_PLAIN, S(stream)));
     HIP_TRY(hrrt::launch_paircol_to_estimate(ctx->dg, view, ctx->pc1, d_prj, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_forward_proj3d_noalloc(hrrt_ctx *ctx, const float *d_image, float *d_prj, int view,
+                                           void *stream) {
+    if (!ctx || !d_image || !d_prj) return fail(HRRT_ERR_ARG, "null argument");
+    if (view < 0 || view >= ctx->geo.views / 2) return fail(HRRT_ERR_ARG, "view %d out of range", view);
+    if (ctx->col_floats() > (size_t)HRRT_NOALLOC_SCRATCH_FLOATS)
+        return fail(HRRT_ERR_ARG, "a pair-column of this geometry (%zu floats) exceeds the %d-float scratch",
+                    ctx->col_floats(), HRRT_NOALLOC_SCRATCH_FLOATS);
+    if (!ctx->noalloc_scratch)
+        return fail(HRRT_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(ctx->noalloc_err));
+    hrrt::PairBatch b{view, 0, 1};
+    HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_image, ctx->noalloc_scratch, hrrt::FpIO(), hrrt::FP_PLAIN, S(stream)));
+    HIP_TRY(hrrt::launch_paircol_to_estimate(ctx->dg, view, ctx->noalloc_scratch, d_prj, S(stream)));
     return HRRT_OK;
 }
 

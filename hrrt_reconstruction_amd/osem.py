@@ -396,6 +396,12 @@ class Osem3D(Layout):
         _lib.call("hrrt_forward_proj3d", self.ctx, src, self._est(prj), int(view), _stream())
         return prj
 
+    def forward_proj3d_noalloc(self, image, view: int, prj):
+        """forward_proj3d without any device allocation: the pair-column staging is the library's
+        static per-GPU scratch (calls that overlap on one GPU race); prj must be given."""
+        _lib.call("hrrt_forward_proj3d_noalloc", self.ctx, self._img(image), self._est(prj), int(view), _stream())
+        return prj
+
     def back_proj3d_incremental(self, prj, ima, view: int):
         _lib.call("hrrt_back_proj3d_incremental", self.ctx, self._est(prj), self._img(ima, "ima"), int(view),
                   _stream())

@@ -252,6 +252,17 @@ int hrrt_timing_read(hrrt_ctx *ctx, double *ms /* [HRRT_NKERNELS] */, int *count
  * pair `view` in the [2S][N] layout.  Bins the reference does not write
  * (xr = 0, columns outside the FOV, masked columns) are left untouched. */
 int hrrt_forward_proj3d(hrrt_ctx *ctx, const float *d_image, float *d_prj, int view, void *stream);
+/* No counterpart in the reference.  hrrt_forward_proj3d for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same result, bit for bit, with no hipMalloc, hipFree or allocating
+ * library call.  Its pair-column staging lives in a file-scope device array of
+ * HRRT_NOALLOC_SCRATCH_FLOATS floats (13.6 MB, every GPU the library runs on holds one for
+ * the life of the process); geometries whose pair-column (2*N*spp floats, 3.35 M for HRRT
+ * span 3 at 256) exceeds it get HRRT_ERR_ARG.  All calls on one GPU share that array: calls
+ * that overlap in time on one GPU (other streams, other contexts, other threads) race and
+ * must be serialised by the caller. */
+#define HRRT_NOALLOC_SCRATCH_FLOATS (2 * 256 * 6656)
+int hrrt_forward_proj3d_noalloc(hrrt_ctx *ctx, const float *d_image, float *d_prj, int view, void *stream);
 /* back_proj3d2 (PJ:1457): rescales d_prj in place (as the reference does),
  * back-projects both views of the pair and adds the rotated result into d_ima. */
 int hrrt_back_proj3d(hrrt_ctx *ctx, float *d_prj, float *d_ima, int view, void *stream);
