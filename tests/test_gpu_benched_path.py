@@ -82,6 +82,34 @@ def test_osem_subsets_c2_bitexact(torch_dev, c2_data, nthreads):
     g.set_backproject_schedule(1)
 
 
+def test_osem_subset_c2_every_forward_shape(torch_dev, c2_data):
+    """The benchmark's subset update under every forward launch shape the tuner may pick
+    (planes per thread, workgroups per ray), bit-exact against the oracle; the plain forward
+    projection is the same under every shape."""
+    from hrrt_reconstruction_amd import autotune
+    o, g, d_pr, d_add, prompt, add, d_nf = c2_data
+    g.set_backproject_schedule(1)
+    s = int(o.order[1])
+    rng = np.random.default_rng(79)
+    img0 = o.init_image() * rng.uniform(0.5, 2.0, (o.N, o.N, o.Z4)).astype(np.float32)
+    ref = img0.copy()
+    print("c2 every shape: oracle subset", flush=True)
+    o.osem_subset(ref, o.image_zeros(), from_dev(d_nf[s]), add, prompt, s, nthreads=1, workers=16)
+    d_img0 = to_dev(img0)
+    fp_ref = None
+    for shape in autotune.CANDIDATES:
+        print(f"c2 forward shape {shape}", flush=True)
+        g.set_launch_shape(*shape)
+        d_img = d_img0.clone()
+        g.osem_subset(d_img, d_nf, d_pr, d_add, s)
+        assert np.array_equal(from_dev(d_img), ref), f"forward shape {shape}"
+        fp = from_dev(g.forward_proj3d(d_img0, 41))
+        if fp_ref is None:
+            fp_ref = fp
+        assert np.array_equal(fp, fp_ref), f"forward projection, shape {shape}"
+    g.set_launch_shape(4, 0)
+
+
 @pytest.mark.parametrize("nthreads", [1, 8])
 def test_osem_full_iteration_c1_bitexact(torch_dev, oracle_lib, nthreads):
     """CalculateOsem3d's whole iteration at span 9, 128x128: 16 subsets in sequence order
