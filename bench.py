@@ -71,20 +71,86 @@ def algorithmic_flops(g, npairs):
     }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2.txt")
+KERNEL_SOURCES = [os.path.join(ROOT, "hrrt_reconstruction_amd", "csrc", f) for f in ("kernels.hip", "kernels.h")]
+
+
+def kernel_source_hash():
+    """sha256 of the kernel sources the traffic figures were measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for path in KERNEL_SOURCES:
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def read_traffic(kernel):
-    """Measured HBM bytes per launch (fetched + written) from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "traffic_c2.txt")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
+    """Measured HBM bytes per launch (fetched + written) from the committed rocprofv3 PMC summary,
+    or (None, reason) when it is missing or was measured on other kernel sources."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, "no traffic summary"
+    src, val = None, None
+    with open(TRAFFIC_FILE) as f:
         for line in f:
             parts = line.split()
-            if len(parts) >= 3 and parts[0] == kernel:
+            if len(parts) == 2 and parts[0] == "source-sha256":
+                src = parts[1]
+            elif len(parts) >= 3 and parts[0] == kernel:
                 try:
-                    return float(parts[1]) + float(parts[2])
+                    val = float(parts[1]) + float(parts[2])
                 except ValueError:
-                    return None
-    return None
+                    val = None
+    if src != kernel_source_hash():
+        return None, "stale: the traffic summary was measured on other kernel sources"
+    return val, None if val is not None else f"no traffic figure for {kernel}"
+
+
+def write_traffic(fetch_dir, write_dir, launches):
+    """Writes profiles/traffic_c2.txt from two rocprofv3 --pmc runs of this bench (FETCH_SIZE,
+    then WRITE_SIZE, in KiB per dispatch): the mean of the last `launches` dispatches of each
+    subset kernel, scaled by what the same counters read for the bench's 256 MiB device copy,
+    with the sha256 of the kernel sources measured."""
+    import csv
+    import glob
+
+    def per_dispatch(d, counter):
+        rows = []
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r["Counter_Name"] == counter:
+                        rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+        rows.sort()
+        return rows
+
+    kinds = {"fp_ratio": "fp_kernel<4, 1, 1, true>", "backproject": "bp_inc_kernel",
+             "rotate_update": "rot_update_kernel<false>"}
+    cal_bytes = 256 << 20
+    out = {}
+    scale = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = per_dispatch(d, counter)
+        copies = [v for _, name, v in rows if "copy" in name.lower()]
+        if not copies:
+            raise SystemExit(f"no copy dispatch in {d}")
+        # the calibration copy is the largest copy dispatch; its counter reads cal_bytes / k KiB
+        k = cal_bytes / 1024 / max(copies)
+        scale[counter] = round(k, 3)
+        for kind, pat in kinds.items():
+            vals = [v for _, name, v in rows if pat in name][-launches:]
+            if not vals:
+                raise SystemExit(f"no {pat} dispatch in {d}")
+            out.setdefault(kind, []).append(sum(vals) / len(vals) * 1024 * k)
+    with open(TRAFFIC_FILE, "w") as f:
+        f.write("# HBM bytes per launch (fetched, written) of the OSEM subset kernels, HRRT span 3 256x256x207,\n")
+        f.write(f"# from rocprofv3 FETCH_SIZE x {scale['FETCH_SIZE']} / WRITE_SIZE x {scale['WRITE_SIZE']} KiB "
+                "(the factors a 256 MiB device copy gives), mean of the last "
+                f"{launches} launches; written by bench.py --write-traffic\n")
+        f.write(f"source-sha256 {kernel_source_hash()}\n")
+        for kind, (fe, wr) in out.items():
+            f.write(f"{kind} {int(round(fe))} {int(round(wr))}\n")
+    print(open(TRAFFIC_FILE).read())
 
 
 def build_study(g, torch, seed):
@@ -148,6 +214,86 @@ def cpu_baseline(g, torch, prompt, add, normfac, image, nsub, threads, schedule=
 
 
 DYN_GEOM = dict(x_pixels=128, span=9, maxdel=67)
+
+
+def cpu_model():
+    """The host CPU's model name (as lscpu reports it), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def c1_leg(torch, seed, iterations, cpu_threads, run_cpu):
+    """The reference's CPU-runnable case: span 9, 128x128x207 (the -X 128 rebinning of the
+    256x288 sinogram), 16 subsets x 1 iteration, OP weighting, one study of 200M prompts with
+    randoms 10 % and scatter 15 %.  GPU: whole iterations (16 subset updates and the
+    end-of-iteration threshold) from the initial image, timed; CPU: one whole iteration of the
+    oracle with the reference's thread schedule for cpu_threads threads, and the GPU iteration
+    under that same schedule compared with it bit for bit."""
+    import numpy as np
+    from hrrt_reconstruction_amd import phantom as ph
+    from hrrt_reconstruction_amd.osem import Osem3D
+    g = Osem3D("cuda:0", **DYN_GEOM)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    pg = torch.Generator()
+    pg.manual_seed(seed + 1)
+    _, st = ph.make_geometry_study(g, 200e6, gen, pg, with_scatter=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prompt, add, invnorm = g.stage(st["prompt"], st["delayed"], st["norm"], st["scatter"], st["atten"])
+    torch.cuda.synchronize()
+    stage_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    normfac = g.calculate_normfac(invnorm)
+    torch.cuda.synchronize()
+    normfac_s = time.perf_counter() - t0
+    del st, invnorm
+    img = g.init_image()
+    g.calculate_osem3d(img, normfac, prompt, add, iterations=1)      # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iterations):
+        img = g.init_image()
+        g.calculate_osem3d(img, normfac, prompt, add, iterations=1)
+    torch.cuda.synchronize()
+    gpu_it = (time.perf_counter() - t0) / iterations
+    log(f"c1: GPU iteration (16 subsets) {gpu_it * 1e3:.2f} ms")
+    out = {"workload": "span 9, 128x128x207, 2209-plane 128x144 sinogram (-X 128 of 256x288), 200M prompts, "
+                       "randoms 10%, scatter 15%, 16 subsets x 1 iteration",
+           "stage_s": round(stage_s, 4), "normfac_s": round(normfac_s, 4),
+           "gpu_iteration_ms": round(gpu_it * 1e3, 3), "gpu_subset_iters_per_s": round(g.subsets / gpu_it, 2),
+           "gpu_backproject_schedule": g.backproject_schedule}
+    if run_cpu:
+        from oracle import oracle
+        oracle.build()
+        o = oracle.Oracle(**DYN_GEOM)
+        pr = paircol_to_sino(g, torch, prompt).cpu().numpy()
+        ad = paircol_to_sino(g, torch, add).cpu().numpy()
+        nfs = [np.ascontiguousarray(normfac[s].cpu().numpy()) for s in range(g.subsets)]
+        ref = o.init_image()
+        corr = o.image_zeros()
+        t0 = time.perf_counter()
+        for i in range(g.subsets):
+            s_ = int(o.order[i])
+            o.osem_subset(ref, corr, nfs[s_], ad, pr, s_, nthreads=cpu_threads, workers=cpu_threads)
+        o.end_iteration(ref)
+        cpu_it = time.perf_counter() - t0
+        log(f"c1: CPU oracle iteration {cpu_it:.2f} s ({cpu_threads} threads)")
+        g.set_backproject_schedule(cpu_threads)
+        img = g.init_image()
+        g.calculate_osem3d(img, normfac, prompt, add, iterations=1)
+        same = bool(np.array_equal(img.cpu().numpy(), ref))
+        out.update({"cpu_iteration_s": round(cpu_it, 3), "cpu_subset_iters_per_s": round(g.subsets / cpu_it, 3),
+                    "cpu_threads": cpu_threads, "cpu_kind": "port", "cpu_model": cpu_model(),
+                    "gpu_image_bitexact_vs_cpu_same_schedule": same})
+    g.close()
+    return out
 
 
 def psf_leg(torch, g, image0, normfac_psf, prompt, add, order, steps):
@@ -523,9 +669,19 @@ def main():
     ap.add_argument("--c5-iterations", type=int, default=3)
     ap.add_argument("--c5-fp32-frames", type=int, default=4,
                     help="frames of that study also run in the FP32 batched and single-frame forms")
+    ap.add_argument("--no-host-stage", dest="host_stage", action="store_false",
+                    help="skip the staging from host memory")
+    ap.add_argument("--c1-iterations", type=int, default=5,
+                    help="timed GPU iterations of the span-9 128x128 case (0 = skip)")
     ap.add_argument("--c5-fp64-frames", type=int, default=2,
                     help="frames of that study checked against the FP64 oracle on the host (0 = skip)")
+    ap.add_argument("--write-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic_c2.txt from two rocprofv3 --pmc output folders and exit")
+    ap.add_argument("--traffic-launches", type=int, default=4)
     args = ap.parse_args()
+    if args.write_traffic:
+        write_traffic(args.write_traffic[0], args.write_traffic[1], args.traffic_launches)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(frames and view pairs would shard across GPUs)", file=sys.stderr)
@@ -554,6 +710,30 @@ def main():
     torch.cuda.synchronize()
     stage_s = time.perf_counter() - t0
     log(f"staged sinograms in {stage_s:.2f} s")
+    stage_host_s = stage_pinned_s = None
+    if args.host_stage:
+        # the same files in host memory: pageable arrays, then pinned tensors (PCIe included)
+        names = ("prompt", "delayed", "norm", "scatter", "atten")
+        host = [study[k].cpu().numpy().ravel() for k in names]
+        g.stage_host(*host)                       # first call sets up the device file copies
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        outs = g.stage_host(*host)
+        torch.cuda.synchronize()
+        stage_host_s = time.perf_counter() - t0
+        same = all(torch.equal(a, b) for a, b in zip(outs, (prompt, add, invnorm)))
+        del outs
+        pinned = [torch.from_numpy(h).pin_memory() for h in host]
+        del host
+        t0 = time.perf_counter()
+        outs = g.stage_host(*pinned)
+        torch.cuda.synchronize()
+        stage_pinned_s = time.perf_counter() - t0
+        same = same and all(torch.equal(a, b) for a, b in zip(outs, (prompt, add, invnorm)))
+        del outs, pinned
+        if not same:
+            raise RuntimeError("staging from host memory differs from staging from device memory")
+        log(f"staged from host memory in {stage_host_s:.3f} s (pageable), {stage_pinned_s:.3f} s (pinned)")
     t0 = time.perf_counter()
     normfac = g.calculate_normfac(invnorm)
     torch.cuda.synchronize()
@@ -599,7 +779,7 @@ def main():
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms = kt[dom][0] / max(kt[dom][1], 1)
     achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
-    traffic = read_traffic(dom)
+    traffic, traffic_note = read_traffic(dom)
     flops = algorithmic_flops(g, npairs)
     dom_tflops = flops[dom] / (dom_ms * 1e-3) / 1e12
     subset_tflops = sum(flops.values()) * value / 1e12
@@ -608,6 +788,7 @@ def main():
     if args.cpu_subsets > 0:
         rate, dt = cpu_baseline(g, torch, prompt, add, normfac, image0, args.cpu_subsets, args.cpu_threads)
         cpu = {"value": round(rate, 4), "unit": "subset-iters/s", "cores": args.cpu_threads, "kind": "port",
+               "model": cpu_model(),
                "sample": f"{args.cpu_subsets} OSEM subset updates of the same workload after one warm-up "
                          f"subset, reference thread schedule T={args.cpu_threads} ({dt:.1f} s)"}
         # the stronger CPU figure: every host core this process may use (at most 16), each
@@ -615,6 +796,7 @@ def main():
         ncores = min(len(os.sched_getaffinity(0)), 16)
         rate2, dt2 = cpu_baseline(g, torch, prompt, add, normfac, image0, args.cpu_subsets, ncores, schedule=1)
         cpu_strong = {"value": round(rate2, 4), "unit": "subset-iters/s", "cores": ncores, "kind": "port",
+                      "model": cpu_model(),
                       "sample": f"{args.cpu_subsets} OSEM subset updates after one warm-up subset, {ncores} "
                                 f"worker threads, every pair incremental ({dt2:.1f} s)"}
 
@@ -632,6 +814,9 @@ def main():
     batched = None
     if args.frames > 0:
         batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
+    c1 = None
+    if args.c1_iterations > 0:
+        c1 = c1_leg(torch, args.seed + 19, args.c1_iterations, args.cpu_threads, args.cpu_subsets > 0)
     schedule = g.backproject_schedule
     c5 = None
     if args.c5_frames > 0:
@@ -658,12 +843,14 @@ def main():
                                "randoms 10%, scatter 15%, 16 subsets (9 view pairs per subset)",
                    "subsets": g.subsets, "view_pairs_per_subset": npairs,
                    "stage_s": round(stage_s, 3), "normfac_s": round(normfac_s, 3),
+                   "stage_host_pageable_s": round(stage_host_s, 3) if stage_host_s else None,
+                   "stage_host_pinned_s": round(stage_pinned_s, 3) if stage_pinned_s else None,
                    "fp_launch_shape": list(shape) if shape else None, "direct_bp_waves": bp_choice,
                    "back_projector": "incremental" if schedule == 1 else ("direct" if schedule == 0 else "mixed"),
                    "backproject_schedule": schedule},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": traffic},
+                     "traffic": traffic, "traffic_note": traffic_note},
         "compute_roofline": {"bound": "fp32", "kernel": dom, "achieved": round(dom_tflops, 3),
                              "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                              "frac": round(dom_tflops / FP32_PEAK_TFLOPS, 5),
@@ -677,6 +864,7 @@ def main():
         "randoms": randoms,
         "listmode": listmode,
         "batched": batched,
+        "c1": c1,
         "c5": c5,
     }
     print(json.dumps(out), flush=True)

@@ -92,6 +92,8 @@ SIGNATURES = {
     "hrrt_osem_subset": (_I, [_VP, _VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_forward_proj3d_noalloc": (_I, [_VP, _VP, _VP, _I, _VP]),
     "hrrt_stage_sinograms_w012": (_I, [_VP, _I, _I] + [_VP] * 8 + [_VP]),
+    "hrrt_stage_sinograms_host": (_I, [_VP] * 9 + [_VP]),
+    "hrrt_stage_sinograms_w012_host": (_I, [_VP, _I, _I] + [_VP] * 8 + [_VP]),
     "hrrt_osem_subset_w012": (_I, [_VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_end_iteration": (_I, [_VP, _VP, _VP]),
     "hrrt_osem_subset_frames": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, _VP]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -215,8 +216,46 @@ struct hrrt_ctx {
         gexec.clear();
         gkey = GraphKey{};
     }
+    // staging from host memory: device copies of the flat files, a pinned bounce ring, a copy
+    // stream, and the events that order them against the staging launches
+    static constexpr int NRING = 4;
+    static constexpr size_t RING_BYTES = 16u << 20;
+    void *hf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // per file kind, 4 bytes per file bin
+    std::vector<int> file_seg;                                     // [th] first file plane of segment th
+    void *ring[NRING] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ring_ev[NRING] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copy_start = nullptr;
+    std::vector<hipEvent_t> seg_ev;
+    int ensure_host_staging() {
+        if (!copy_stream) HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+        if (!copy_start) HIP_TRY(hipEventCreateWithFlags(&copy_start, hipEventDisableTiming));
+        for (int i = 0; i < NRING; i++) {
+            if (!ring[i] && hipHostMalloc(&ring[i], RING_BYTES, hipHostMallocDefault) != hipSuccess) {
+                ring[i] = nullptr;
+                return fail(HRRT_ERR_NOMEM, "hipHostMalloc(%zu) of a staging buffer failed", RING_BYTES);
+            }
+            if (!ring_ev[i]) HIP_TRY(hipEventCreateWithFlags(&ring_ev[i], hipEventDisableTiming));
+        }
+        while ((int)seg_ev.size() < geo.th_pixels) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            seg_ev.push_back(e);
+        }
+        return HRRT_OK;
+    }
     ~hrrt_ctx() {
         drop_graphs();
+        if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+        for (void *p : hf)
+            if (p) (void)hipFree(p);
+        for (int i = 0; i < NRING; i++) {
+            if (ring[i]) (void)hipHostFree(ring[i]);
+            if (ring_ev[i]) (void)hipEventDestroy(ring_ev[i]);
+        }
+        for (hipEvent_t e : seg_ev) (void)hipEventDestroy(e);
+        if (copy_start) (void)hipEventDestroy(copy_start);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
         void *s[] = {sub_ratio, bufA, bufB, pc1, colmask, gapmask, h_img, h_prj, h_buf, rot_zero, (void *)dg.nzmask, f_ratio, fA, fB,
@@ -391,6 +430,7 @@ extern "C" int hrrt_create(const hrrt_params *params, hrrt_ctx **out) {
         const int sw = s % 2 == 1 ? s + 1 : (s == 0 ? 0 : s - 1);   // FIO:116-118
         file_seg[s] = G.seg_offset[sw] / (N * G.views);
     }
+    ctx->file_seg = file_seg;
     std::vector<float> resc = G.rescalearray;
     // back-projection column in LDS: segment 0, then every oblique segment inside a window of
     // zero planes wide enough that the back projector's 64-plane slots 0-2 (z < 192) always,
@@ -667,7 +707,8 @@ extern "C" int hrrt_stage_sinograms(hrrt_ctx *ctx, const int16_t *d_prompt_file,
     if (d_norm_file && G.x_rebin * G.v_rebin > 1)
         if ((r = ctx->zalloc((void **)&ctx->gapmask, (size_t)G.th_pixels * G.p.def_angles * G.p.def_elements))) return r;
     HIP_TRY(hrrt::launch_stage(ctx->dg, d_prompt_file, d_delayed_file, d_norm_file, d_scatter_file, d_atten_file,
-                               d_prompt, d_add, d_invnorm, ctx->gapmask, ctx->colmask, S(stream)));
+                               d_prompt, d_add, d_invnorm, ctx->gapmask, ctx->colmask, 0, G.th_pixels, 0,
+                               G.segmentsize, true, S(stream)));
     return HRRT_OK;
 }
 
@@ -690,8 +731,125 @@ extern "C" int hrrt_stage_sinograms_w012(hrrt_ctx *ctx, int weighting, int preco
         if ((r = ctx->zalloc((void **)&ctx->gapmask, (size_t)G.th_pixels * G.p.def_angles * G.p.def_elements))) return r;
     HIP_TRY(hrrt::launch_stage_w012(ctx->dg, weighting, precorrected != 0, d_data16_file, d_dataf_file, d_delayed_file,
                                     d_norm_file, d_scatter_file, d_atten_file, d_data, d_invnorm, ctx->gapmask,
-                                    ctx->colmask, S(stream)));
+                                    ctx->colmask, 0, G.th_pixels, 0, G.segmentsize, true, S(stream)));
     return HRRT_OK;
+}
+
+// Staging from host memory (the reference reads the flat files segment by segment,
+// FIO:110-259).  Every segment's block of each given file (file planes file_seg[th] ..
+// + planes of th, contiguous) goes to the context's device copy of that file on a copy stream
+// -- straight from the caller's memory when it is pinned, else through a ring of pinned
+// buffers filled on this thread while earlier blocks are in flight -- and the staging launch
+// of that segment waits for its blocks only, so copies of later segments overlap the staging
+// of earlier ones.  Returns when every copy has completed (the host files may then be
+// reused); the staging launches may still run on the caller's stream.
+struct HostFile {
+    const void *src;   // host memory of the whole flat file, or null (file absent)
+    size_t elem;       // bytes per element
+};
+
+static int stage_host(hrrt_ctx *ctx, HostFile files[5], void *dev[5], hipStream_t s,
+                      const std::function<hipError_t(int th, bool finish)> &launch) {
+    const hrrt::Geometry &G = ctx->geo;
+    int r;
+    if ((r = ctx->ensure_host_staging())) return r;
+    const size_t plane = (size_t)G.p.def_angles * G.p.def_elements;
+    bool pinned[5];
+    for (int f = 0; f < 5; f++) {
+        dev[f] = nullptr;
+        pinned[f] = false;
+        if (!files[f].src) continue;
+        const size_t bytes = (size_t)G.segmentsize * plane * sizeof(float);   // short or float files
+        if (!ctx->hf[f] && hipMalloc(&ctx->hf[f], bytes) != hipSuccess) {
+            ctx->hf[f] = nullptr;
+            return fail(HRRT_ERR_NOMEM, "hipMalloc(%zu) of a device file copy failed", bytes);
+        }
+        dev[f] = ctx->hf[f];
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, files[f].src) == hipSuccess && at.type == hipMemoryTypeHost) pinned[f] = true;
+        (void)hipGetLastError();   // pageable memory reports an error here: not one of ours
+    }
+    // the device copies may still be read by an earlier staging on the caller's stream
+    HIP_TRY(hipEventRecord(ctx->copy_start, s));
+    HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->copy_start, 0));
+    int slot = 0;
+    for (int th = 0; th < G.th_pixels; th++) {
+        const size_t np = (size_t)(G.yr_top[th] - G.yr_bottom[th] + 1);
+        for (int f = 0; f < 5; f++) {
+            if (!files[f].src) continue;
+            const size_t off = (size_t)ctx->file_seg[th] * plane * files[f].elem, n = np * plane * files[f].elem;
+            const char *src = static_cast<const char *>(files[f].src) + off;
+            char *dst = static_cast<char *>(dev[f]) + off;
+            if (pinned[f]) {
+                HIP_TRY(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, ctx->copy_stream));
+                continue;
+            }
+            for (size_t done = 0; done < n; done += hrrt_ctx::RING_BYTES) {
+                const size_t c = std::min(hrrt_ctx::RING_BYTES, n - done);
+                HIP_TRY(hipEventSynchronize(ctx->ring_ev[slot]));   // the slot's previous copy is done
+                std::memcpy(ctx->ring[slot], src + done, c);
+                HIP_TRY(hipMemcpyAsync(dst + done, ctx->ring[slot], c, hipMemcpyHostToDevice, ctx->copy_stream));
+                HIP_TRY(hipEventRecord(ctx->ring_ev[slot], ctx->copy_stream));
+                slot = (slot + 1) % hrrt_ctx::NRING;
+            }
+        }
+        HIP_TRY(hipEventRecord(ctx->seg_ev[th], ctx->copy_stream));
+        HIP_TRY(hipStreamWaitEvent(s, ctx->seg_ev[th], 0));
+        HIP_TRY(launch(th, th == G.th_pixels - 1));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_stage_sinograms_host(hrrt_ctx *ctx, const int16_t *prompt_file, const float *delayed_file,
+                                         const float *norm_file, const float *scatter_file, const float *atten_file,
+                                         int16_t *d_prompt, float *d_add, float *d_invnorm, void *stream) {
+    if (!ctx || !d_prompt || !d_add || !d_invnorm) return fail(HRRT_ERR_ARG, "null argument");
+    const hrrt::Geometry &G = ctx->geo;
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->colmask, sizeof(int) * G.views * G.N))) return r;
+    HIP_TRY(hipMemsetAsync(ctx->colmask, 0, sizeof(int) * G.views * G.N, S(stream)));
+    if (norm_file && G.x_rebin * G.v_rebin > 1)
+        if ((r = ctx->zalloc((void **)&ctx->gapmask, (size_t)G.th_pixels * G.p.def_angles * G.p.def_elements))) return r;
+    HostFile files[5] = {{prompt_file, 2}, {delayed_file, 4}, {norm_file, 4}, {scatter_file, 4}, {atten_file, 4}};
+    void *dev[5];
+    return stage_host(ctx, files, dev, S(stream), [&](int th, bool finish) {
+        const int p0 = G.segment_offset[th], p1 = p0 + G.yr_top[th] - G.yr_bottom[th] + 1;
+        return hrrt::launch_stage(ctx->dg, (const int16_t *)dev[0], (const float *)dev[1], (const float *)dev[2],
+                                  (const float *)dev[3], (const float *)dev[4], d_prompt, d_add, d_invnorm,
+                                  ctx->gapmask, ctx->colmask, th, 1, p0, p1, finish, S(stream));
+    });
+}
+
+extern "C" int hrrt_stage_sinograms_w012_host(hrrt_ctx *ctx, int weighting, int precorrected,
+                                              const int16_t *data16_file, const float *dataf_file,
+                                              const float *delayed_file, const float *norm_file,
+                                              const float *scatter_file, const float *atten_file, float *d_data,
+                                              float *d_invnorm, void *stream) {
+    if (!ctx || !d_data || !d_invnorm) return fail(HRRT_ERR_ARG, "null argument");
+    if (weighting < 0 || weighting > 2) return fail(HRRT_ERR_ARG, "weighting %d is not 0 (UW), 1 (AW) or 2 (ANW)", weighting);
+    if (!data16_file == !dataf_file) return fail(HRRT_ERR_ARG, "give exactly one of the short and the float data file");
+    if (precorrected && !dataf_file) return fail(HRRT_ERR_ARG, "a precorrected sinogram is a float file");
+    if (weighting == 0 && !precorrected && scatter_file && !norm_file)
+        return fail(HRRT_ERR_UNSUPPORTED, "-W 0 with scatter and no norm: the reference subtracts a buffer it never "
+                                          "filled (hrrt_osem3d_span3.cpp:1233)");
+    const hrrt::Geometry &G = ctx->geo;
+    int r;
+    if ((r = ctx->zalloc((void **)&ctx->colmask, sizeof(int) * G.views * G.N))) return r;
+    HIP_TRY(hipMemsetAsync(ctx->colmask, 0, sizeof(int) * G.views * G.N, S(stream)));
+    if (norm_file && G.x_rebin * G.v_rebin > 1)
+        if ((r = ctx->zalloc((void **)&ctx->gapmask, (size_t)G.th_pixels * G.p.def_angles * G.p.def_elements))) return r;
+    HostFile files[5] = {{data16_file ? (const void *)data16_file : (const void *)dataf_file, data16_file ? 2u : 4u},
+                         {delayed_file, 4}, {norm_file, 4}, {scatter_file, 4}, {atten_file, 4}};
+    void *dev[5];
+    return stage_host(ctx, files, dev, S(stream), [&](int th, bool finish) {
+        const int p0 = G.segment_offset[th], p1 = p0 + G.yr_top[th] - G.yr_bottom[th] + 1;
+        return hrrt::launch_stage_w012(ctx->dg, weighting, precorrected != 0,
+                                       data16_file ? (const int16_t *)dev[0] : nullptr,
+                                       data16_file ? nullptr : (const float *)dev[0], (const float *)dev[1],
+                                       (const float *)dev[2], (const float *)dev[3], (const float *)dev[4], d_data,
+                                       d_invnorm, ctx->gapmask, ctx->colmask, th, 1, p0, p1, finish, S(stream));
+    });
 }
 
 // Back projection of the pairs of a batch under the reference's thread schedule

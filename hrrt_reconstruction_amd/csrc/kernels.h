@@ -113,16 +113,19 @@ hipError_t launch_convolve3d(const DevGeom &g, const float *in, float *out, floa
                              const float *taps_y, int hy, const float *normz, int z_min, int z_max, hipStream_t s);
 hipError_t launch_init_image(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s);
 hipError_t launch_end_iteration(const DevGeom &g, float *image, int z_min, int z_max, hipStream_t s);
+// Segments [th0, th0 + nth), whose internal planes are [p_lo, p_hi) (their file blocks must be
+// in place); finish: derive the zero-column mask (after the last segment).
 hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *delayed_f, const float *norm_f,
                         const float *scatter_f, const float *atten_f, int16_t *prompt, float *add, float *invnorm,
-                        uint8_t *gapmask, int *colmask, hipStream_t s);
+                        uint8_t *gapmask, int *colmask, int th0, int nth, int p_lo, int p_hi, bool finish,
+                        hipStream_t s);
 // staging for the UW/AW/ANW weightings (weighting 0, 1, 2): data_f16 (short prompts or trues)
 // or data_ff (float, or fully precorrected when precor) -> data (the w012 numerator) and the
 // sensitivity weights of that weighting
 hipError_t launch_stage_w012(const DevGeom &g, int weighting, int precor, const int16_t *data_f16,
                              const float *data_ff, const float *delayed_f, const float *norm_f, const float *scatter_f,
                              const float *atten_f, float *data, float *invnorm, uint8_t *gapmask, int *colmask,
-                             hipStream_t s);
+                             int th0, int nth, int p_lo, int p_hi, bool finish, hipStream_t s);
 
 // Records msg as hrrt_last_error()'s message of this thread and returns code.
 int set_last_error(int code, const char *msg);

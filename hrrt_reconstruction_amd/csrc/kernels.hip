@@ -1191,6 +1191,7 @@ struct StageArgs {
     int weighting = 3, precor = 0;
     const float *dataf_f = nullptr;
     float *data = nullptr;
+    int p_lo = 0, p_hi = 0;   // internal planes [p_lo, p_hi) staged by this launch
 };
 
 __device__ __forceinline__ float rebin_sum(const DevGeom &g, const float *f, size_t fplane, int v, int col) {
@@ -1254,8 +1255,8 @@ __device__ __forceinline__ int16_t rebin_short(const DevGeom &g, const int16_t *
 }
 
 // mask of file (view, bin) with a positive norm in some non-killed plane of segment th (FIO:158-160)
-__global__ void gapmask_kernel(const DevGeom g, const float *__restrict__ norm_f, uint8_t *__restrict__ gm) {
-    const int th = blockIdx.y;
+__global__ void gapmask_kernel(const DevGeom g, const float *__restrict__ norm_f, uint8_t *__restrict__ gm, int th0) {
+    const int th = th0 + blockIdx.y;
     const size_t fp = (size_t)g.def_angles * g.def_elements;
     const int np = g.yt[th] - g.yb[th] + 1;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < fp; i += (size_t)gridDim.x * blockDim.x) {
@@ -1270,8 +1271,8 @@ __global__ void gapmask_kernel(const DevGeom g, const float *__restrict__ norm_f
 
 __global__ void stage_kernel(const DevGeom g, StageArgs a) {
     const int N = g.N, V = g.views;
-    const size_t total = (size_t)g.S * V * N;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+    const size_t total = (size_t)a.p_hi * V * N;
+    for (size_t idx = (size_t)a.p_lo * V * N + (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (size_t)gridDim.x * blockDim.x) {
         const int col = (int)(idx % N);
         const int v = (int)((idx / N) % V);
@@ -1661,35 +1662,41 @@ hipError_t launch_end_iteration(const DevGeom &g, float *image, int z_min, int z
     return hipGetLastError();
 }
 
+// Segments [th0, th0 + nth) with internal planes [p_lo, p_hi): their gap masks first when the
+// norm is rebinned, then their planes; finish derives the zero-column mask from the column
+// flags of every segment staged so far.
+static void stage_launch(const DevGeom &g, StageArgs a, int th0, int nth, int p_lo, int p_hi, bool finish,
+                         hipStream_t s) {
+    a.p_lo = p_lo;
+    a.p_hi = p_hi;
+    if (a.norm_f && g.x_rebin * g.v_rebin > 1 && nth > 0)
+        hipLaunchKernelGGL(gapmask_kernel, dim3(256, nth), dim3(256), 0, s, g, a.norm_f, const_cast<uint8_t *>(a.gapmask), th0);
+    const size_t n = (size_t)(p_hi - p_lo) * g.views * g.N;
+    const int blocks = (int)std::min<size_t>(8192, (n + 255) / 256);
+    if (blocks > 0) hipLaunchKernelGGL(stage_kernel, dim3(blocks), dim3(256), 0, s, g, a);
+    if (finish) hipLaunchKernelGGL(nzmask_kernel, dim3(g.views / 2), dim3(256), 0, s, g, a.colmask, g.nzmask);
+}
+
 hipError_t launch_stage(const DevGeom &g, const int16_t *prompt_f, const float *delayed_f, const float *norm_f,
                         const float *scatter_f, const float *atten_f, int16_t *prompt, float *add, float *invnorm,
-                        uint8_t *gapmask, int *colmask, hipStream_t s) {
-    const int rebin = g.x_rebin * g.v_rebin;
-    if (norm_f && rebin > 1) {
-        hipLaunchKernelGGL(gapmask_kernel, dim3(256, g.th), dim3(256), 0, s, g, norm_f, gapmask);
-    }
+                        uint8_t *gapmask, int *colmask, int th0, int nth, int p_lo, int p_hi, bool finish,
+                        hipStream_t s) {
     StageArgs a{prompt_f, delayed_f, norm_f, scatter_f, atten_f, prompt, add, invnorm, gapmask, colmask, g.plane_seg};
-    hipLaunchKernelGGL(stage_kernel, dim3(8192), dim3(256), 0, s, g, a);
-    hipLaunchKernelGGL(nzmask_kernel, dim3(g.views / 2), dim3(256), 0, s, g, colmask, g.nzmask);
+    stage_launch(g, a, th0, nth, p_lo, p_hi, finish, s);
     return hipGetLastError();
 }
 
 hipError_t launch_stage_w012(const DevGeom &g, int weighting, int precor, const int16_t *data_f16,
                              const float *data_ff, const float *delayed_f, const float *norm_f, const float *scatter_f,
                              const float *atten_f, float *data, float *invnorm, uint8_t *gapmask, int *colmask,
-                             hipStream_t s) {
-    const int rebin = g.x_rebin * g.v_rebin;
-    if (norm_f && rebin > 1) {
-        hipLaunchKernelGGL(gapmask_kernel, dim3(256, g.th), dim3(256), 0, s, g, norm_f, gapmask);
-    }
+                             int th0, int nth, int p_lo, int p_hi, bool finish, hipStream_t s) {
     StageArgs a{data_f16, delayed_f, norm_f, scatter_f, atten_f, nullptr, nullptr, invnorm, gapmask, colmask,
                 g.plane_seg};
     a.weighting = weighting;
     a.precor = precor;
     a.dataf_f = data_ff;
     a.data = data;
-    hipLaunchKernelGGL(stage_kernel, dim3(8192), dim3(256), 0, s, g, a);
-    hipLaunchKernelGGL(nzmask_kernel, dim3(g.views / 2), dim3(256), 0, s, g, colmask, g.nzmask);
+    stage_launch(g, a, th0, nth, p_lo, p_hi, finish, s);
     return hipGetLastError();
 }
 

@@ -139,6 +139,30 @@ class Osem3D(Layout):
             raise ValueError(f"{name} holds {t.numel()} elements, expected {numel}")
         return C.c_void_p(t.data_ptr())
 
+    @staticmethod
+    def _h(a, dtype, numel, name, optional=False):
+        """Host pointer of a numpy array or CPU tensor after checking dtype, layout and size."""
+        if a is None:
+            if optional:
+                return C.c_void_p(0)
+            raise ValueError(f"{name} is required")
+        np_dtype = {torch.int16: np.int16, torch.float32: np.float32}[dtype]
+        if isinstance(a, torch.Tensor):
+            if a.device.type != "cpu":
+                raise ValueError(f"{name} must be in host memory, not on {a.device}")
+            if a.dtype != dtype or not a.is_contiguous():
+                raise TypeError(f"{name} must be a contiguous {dtype} tensor")
+            n, ptr = a.numel(), a.data_ptr()
+        elif isinstance(a, np.ndarray):
+            if a.dtype != np_dtype or not a.flags["C_CONTIGUOUS"]:
+                raise TypeError(f"{name} must be a C-contiguous {np.dtype(np_dtype).name} array")
+            n, ptr = a.size, a.ctypes.data
+        else:
+            raise TypeError(f"{name} must be a numpy array or a CPU tensor, not {type(a).__name__}")
+        if n != numel:
+            raise ValueError(f"{name} holds {n} elements, expected {numel}")
+        return C.c_void_p(ptr)
+
     def _img(self, t, name="image", n=1):
         return self._t(t, torch.float32, n * self.image_floats, name)
 
@@ -206,6 +230,41 @@ class Osem3D(Layout):
         invnorm = self.paircol_zeros()
         _lib.call("hrrt_stage_sinograms_w012", self.ctx, int(weighting), int(bool(precorrected)), *args, _p(data),
                   _p(invnorm), _stream())
+        return data, invnorm
+
+    def stage_host(self, prompt_file, delayed_file, norm_file, scatter_file, atten_file):
+        """stage() from flat files in host memory (numpy arrays or CPU tensors, pinned or not),
+        copied segment by segment with the staging of each segment overlapping later copies
+        (read_flat_* per segment, file_io_processor.cpp:110-259).  Returns device buffers as
+        stage() does; the host files may be reused when this returns."""
+        fb = self.file_bins
+        args = [self._h(prompt_file, torch.int16, fb, "prompt_file"),
+                self._h(delayed_file, torch.float32, fb, "delayed_file", optional=True),
+                self._h(norm_file, torch.float32, fb, "norm_file", optional=True),
+                self._h(scatter_file, torch.float32, fb, "scatter_file", optional=True),
+                self._h(atten_file, torch.float32, fb, "atten_file", optional=True)]
+        prompt = self.paircol_zeros(torch.int16)
+        add = self.paircol_zeros()
+        invnorm = self.paircol_zeros()
+        _lib.call("hrrt_stage_sinograms_host", self.ctx, *args, _p(prompt), _p(add), _p(invnorm), _stream())
+        return prompt, add, invnorm
+
+    def stage_w012_host(self, weighting: int, data_file, delayed_file=None, norm_file=None, scatter_file=None,
+                        atten_file=None, precorrected: bool = False):
+        """stage_w012() from flat files in host memory (see stage_host)."""
+        fb = self.file_bins
+        short = (data_file.dtype == torch.int16) if isinstance(data_file, torch.Tensor) else \
+            (getattr(data_file, "dtype", None) == np.int16)
+        args = [self._h(data_file, torch.int16, fb, "data_file") if short else C.c_void_p(0),
+                C.c_void_p(0) if short else self._h(data_file, torch.float32, fb, "data_file"),
+                self._h(delayed_file, torch.float32, fb, "delayed_file", optional=True),
+                self._h(norm_file, torch.float32, fb, "norm_file", optional=True),
+                self._h(scatter_file, torch.float32, fb, "scatter_file", optional=True),
+                self._h(atten_file, torch.float32, fb, "atten_file", optional=True)]
+        data = self.paircol_zeros()
+        invnorm = self.paircol_zeros()
+        _lib.call("hrrt_stage_sinograms_w012_host", self.ctx, int(weighting), int(bool(precorrected)), *args,
+                  _p(data), _p(invnorm), _stream())
         return data, invnorm
 
     def norm_zero_mask(self):

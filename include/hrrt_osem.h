@@ -145,6 +145,24 @@ int hrrt_stage_sinograms_w012(hrrt_ctx *ctx, int weighting, int precorrected, co
                               const float *d_scatter_file, const float *d_atten_file, float *d_data,
                               float *d_invnorm, void *stream);
 
+/* The same two stagings from files in host memory, as the reference reads them
+ * (read_flat_float / read_flat_short per segment, FIO:110-259): the file pointers are host
+ * memory (pageable, or pinned -- hipHostMalloc, torch pin_memory -- which is copied
+ * without a bounce), the outputs device memory.  Each segment's block of every file is
+ * copied on the context's own copy stream into a device copy of that file kept in the
+ * context (through a ring of pinned buffers for pageable memory), and that segment's
+ * staging launch on `stream` waits for its blocks only, so later copies overlap earlier
+ * staging.  Returns when every copy has completed: the host files may be reused then;
+ * the staging itself may still be running on `stream`.  Results equal the device-memory
+ * entry points bit for bit. */
+int hrrt_stage_sinograms_host(hrrt_ctx *ctx, const int16_t *prompt_file, const float *delayed_file,
+                              const float *norm_file, const float *scatter_file, const float *atten_file,
+                              int16_t *d_prompt, float *d_add, float *d_invnorm, void *stream);
+int hrrt_stage_sinograms_w012_host(hrrt_ctx *ctx, int weighting, int precorrected, const int16_t *data16_file,
+                                   const float *dataf_file, const float *delayed_file, const float *norm_file,
+                                   const float *scatter_file, const float *atten_file, float *d_data,
+                                   float *d_invnorm, void *stream);
+
 /* ---------------- sensitivity (CalculateNormfac S3:3760-3865) ---------------- */
 /* For every subset: back-project the subset's view pairs of d_invnorm (BP
  * rescale applied on the fly, d_invnorm itself is not modified), rotate and
