@@ -134,8 +134,9 @@ def write_traffic(fetch_dir, write_dir, launches):
         copies = [v for _, name, v in rows if "copy" in name.lower()]
         if not copies:
             raise SystemExit(f"no copy dispatch in {d}")
-        # the calibration copy is the largest copy dispatch; its counter reads cal_bytes / k KiB
-        k = cal_bytes / 1024 / max(copies)
+        # the calibration copy is the run's first copy dispatch (the bench makes it before
+        # anything else); its counter reads cal_bytes / k KiB
+        k = cal_bytes / 1024 / copies[0]
         scale[counter] = round(k, 3)
         for kind, pat in kinds.items():
             vals = [v for _, name, v in rows if pat in name][-launches:]
