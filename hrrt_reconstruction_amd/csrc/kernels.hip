@@ -1269,104 +1269,133 @@ __global__ void gapmask_kernel(const DevGeom g, const float *__restrict__ norm_f
     }
 }
 
-__global__ void stage_kernel(const DevGeom g, StageArgs a) {
-    const int N = g.N, V = g.views;
-    const size_t total = (size_t)a.p_hi * V * N;
-    for (size_t idx = (size_t)a.p_lo * V * N + (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        const int col = (int)(idx % N);
-        const int v = (int)((idx / N) % V);
-        const int p = (int)(idx / ((size_t)N * V));
-        const int th = a.plane_seg[p];
-        const int i = p - g.segment_offset[th];
-        const int np = g.yt[th] - g.yb[th] + 1;
-        const size_t fplane = (size_t)g.file_seg[th] + i;
-        const int rebin = g.x_rebin * g.v_rebin;
-        // norm (read with norm_flag = 1, FIO:110-180)
-        float nrm = 1.0f;
-        if (a.norm_f) {
-            if (i < g.npskip || i + g.npskip >= np) {
-                nrm = 0.0f;
-            } else if (rebin == 1) {
-                nrm = rebin_sum(g, a.norm_f, fplane, v, col);
-            } else {
-                nrm = rebin_norm(g, a.norm_f, fplane, v, col);
-                const size_t fp = (size_t)g.def_angles * g.def_elements;
-                int cnt = 0;
-                for (int vr = 0; vr < g.v_rebin; vr++)
-                    for (int xr = 0; xr < g.x_rebin; xr++)
-                        cnt += a.gapmask[th * fp + (size_t)(v * g.v_rebin + vr) * g.def_elements + col * g.x_rebin + xr];
-                if (cnt < rebin) nrm = 0.0f;
-            }
-        }
-        const float d = a.delayed_f ? rebin_sum(g, a.delayed_f, fplane, v, col) : 0.0f;
-        const float at = a.atten_f ? rebin_sum(g, a.atten_f, fplane, v, col) : 1.0f;
-        if (nrm != 0.0f) atomicOr(&a.colmask[v * N + col], 1);
-        const int pair = v % (V / 2), h = v / (V / 2);
-        const size_t o = ((size_t)(pair * 2 + h) * N + col) * g.spp + g.seg_data[th] + i;
-        if (a.weighting < 3) {
-            // prepare_osem3dw0 S3:1198-1256, w1 S3:1346-1407 and w2 S3:1505-1546; precorrected data
-            // S3:1309-1323, 1410-1482, 1548-1636; then setnegative2zero simd_operation.cpp:78-96
-            float t;
-            if (!a.precor) {
-                t = a.prompt_f ? rebin_short_f(g, a.prompt_f, fplane, v, col) : rebin_sum(g, a.dataf_f, fplane, v, col);
-                if (a.delayed_f) t = t - d;                               // readsinogram S3:1124-1131
-                const float sc = a.scatter_f ? rebin_sum(g, a.scatter_f, fplane, v, col) : 0.0f;
-                if (a.weighting == 0) {
-                    if (a.norm_f) {
-                        t = t * nrm;                                      // S3:1224, 1244
-                        if (a.scatter_f) t = t - nrm;                     // S3:1233: subtracts the norm buffer
-                    }
-                    if (a.atten_f) t = t * at;                            // S3:1253
-                } else if (a.weighting == 1) {
-                    if (a.scatter_f) {
-                        if (a.norm_f) t = nrm > 0.0f ? t - sc / nrm : 0.0f;   // S3:1380-1382
-                        else t = t - sc;                                      // S3:1389
-                    }
-                    if (a.norm_f) t = t * nrm;                                // S3:1405
-                } else if (a.scatter_f) {
-                    t = nrm > 0.0f ? t - sc / nrm : 0.0f;                     // S3:1535-1537
-                }
-            } else {
-                t = rebin_sum(g, a.dataf_f, fplane, v, col);
-                if (a.weighting == 2) t = nrm > 0.0f ? t / nrm : 0.0f;        // S3:1564-1566
-                if (a.weighting >= 1 && a.atten_f && at > 0.0f) t = t / at;  // divideprojection simd_operation.cpp:34
-            }
-            a.data[o] = t > 0.0f ? t : 0.0f;
-            float inv;
-            if (a.weighting == 0) {                                           // norm_UW S3:756-771
-                inv = nrm > 0.0f ? 1.0f : nrm;
-            } else if (a.weighting == 1) {                                    // norm_AW S3:774-791
-                inv = nrm > 0.0f ? 1.0f / (at < 1.0f ? 1.0f : at) : 0.0f;
-            } else {                                                          // norm_ANW S3:796-821
-                const float am = at < 1.0f ? 1.0f : at;
-                if (nrm > 0.0f) {
-                    const float den = nrm * am;
-                    inv = den > g.eps2 ? 1.0f / den : 0.0f;
-                } else {
-                    inv = nrm;
-                }
-            }
-            a.invnorm[o] = inv;
-            continue;
-        }
-        int16_t pr = 0;
-        if (a.prompt_f) pr = rebin_short(g, a.prompt_f, fplane, v, col);
-        float ad = d * nrm;                                     // mulprojection S3:1715
-        if (nrm == 0.0f) pr = 0;                                // S3:1720-1724
-        if (a.scatter_f) ad = ad + rebin_sum(g, a.scatter_f, fplane, v, col);  // addprojection S3:1753
-        ad = ad * at;                                           // mulprojection S3:1769
-        float inv;                                              // norm_ANW S3:809-817
-        const float am = at < 1.0f ? 1.0f : at;
-        if (nrm > 0.0f) {
-            const float den = nrm * am;
-            inv = den > g.eps2 ? 1.0f / den : 0.0f;
+// One element (internal plane p, view v, bin col) of the staging: the corrected data or the
+// additive term (o1), the sensitivity weight (o_inv) and, for OP, the prompt (o_pr); flags the
+// column when its norm is non-zero.
+__device__ __forceinline__ void stage_one(const DevGeom &g, const StageArgs &a, int p, int v, int col, float &o1,
+                                          float &o_inv, int16_t &o_pr) {
+    const int th = a.plane_seg[p];
+    const int i = p - g.segment_offset[th];
+    const int np = g.yt[th] - g.yb[th] + 1;
+    const size_t fplane = (size_t)g.file_seg[th] + i;
+    const int rebin = g.x_rebin * g.v_rebin;
+    // norm (read with norm_flag = 1, FIO:110-180)
+    float nrm = 1.0f;
+    if (a.norm_f) {
+        if (i < g.npskip || i + g.npskip >= np) {
+            nrm = 0.0f;
+        } else if (rebin == 1) {
+            nrm = rebin_sum(g, a.norm_f, fplane, v, col);
         } else {
-            inv = nrm;
+            nrm = rebin_norm(g, a.norm_f, fplane, v, col);
+            const size_t fp = (size_t)g.def_angles * g.def_elements;
+            int cnt = 0;
+            for (int vr = 0; vr < g.v_rebin; vr++)
+                for (int xr = 0; xr < g.x_rebin; xr++)
+                    cnt += a.gapmask[th * fp + (size_t)(v * g.v_rebin + vr) * g.def_elements + col * g.x_rebin + xr];
+            if (cnt < rebin) nrm = 0.0f;
         }
-        a.prompt[o] = pr;
-        a.add[o] = ad;
-        a.invnorm[o] = inv;
+    }
+    const float d = a.delayed_f ? rebin_sum(g, a.delayed_f, fplane, v, col) : 0.0f;
+    const float at = a.atten_f ? rebin_sum(g, a.atten_f, fplane, v, col) : 1.0f;
+    if (nrm != 0.0f) atomicOr(&a.colmask[v * g.N + col], 1);
+    if (a.weighting < 3) {
+        // prepare_osem3dw0 S3:1198-1256, w1 S3:1346-1407 and w2 S3:1505-1546; precorrected data
+        // S3:1309-1323, 1410-1482, 1548-1636; then setnegative2zero simd_operation.cpp:78-96
+        float t;
+        if (!a.precor) {
+            t = a.prompt_f ? rebin_short_f(g, a.prompt_f, fplane, v, col) : rebin_sum(g, a.dataf_f, fplane, v, col);
+            if (a.delayed_f) t = t - d;                               // readsinogram S3:1124-1131
+            const float sc = a.scatter_f ? rebin_sum(g, a.scatter_f, fplane, v, col) : 0.0f;
+            if (a.weighting == 0) {
+                if (a.norm_f) {
+                    t = t * nrm;                                      // S3:1224, 1244
+                    if (a.scatter_f) t = t - nrm;                     // S3:1233: subtracts the norm buffer
+                }
+                if (a.atten_f) t = t * at;                            // S3:1253
+            } else if (a.weighting == 1) {
+                if (a.scatter_f) {
+                    if (a.norm_f) t = nrm > 0.0f ? t - sc / nrm : 0.0f;   // S3:1380-1382
+                    else t = t - sc;                                      // S3:1389
+                }
+                if (a.norm_f) t = t * nrm;                                // S3:1405
+            } else if (a.scatter_f) {
+                t = nrm > 0.0f ? t - sc / nrm : 0.0f;                     // S3:1535-1537
+            }
+        } else {
+            t = rebin_sum(g, a.dataf_f, fplane, v, col);
+            if (a.weighting == 2) t = nrm > 0.0f ? t / nrm : 0.0f;        // S3:1564-1566
+            if (a.weighting >= 1 && a.atten_f && at > 0.0f) t = t / at;  // divideprojection simd_operation.cpp:34
+        }
+        o1 = t > 0.0f ? t : 0.0f;
+        float inv;
+        if (a.weighting == 0) {                                           // norm_UW S3:756-771
+            inv = nrm > 0.0f ? 1.0f : nrm;
+        } else if (a.weighting == 1) {                                    // norm_AW S3:774-791
+            inv = nrm > 0.0f ? 1.0f / (at < 1.0f ? 1.0f : at) : 0.0f;
+        } else {                                                          // norm_ANW S3:796-821
+            const float am = at < 1.0f ? 1.0f : at;
+            if (nrm > 0.0f) {
+                const float den = nrm * am;
+                inv = den > g.eps2 ? 1.0f / den : 0.0f;
+            } else {
+                inv = nrm;
+            }
+        }
+        o_inv = inv;
+        return;
+    }
+    int16_t pr = 0;
+    if (a.prompt_f) pr = rebin_short(g, a.prompt_f, fplane, v, col);
+    float ad = d * nrm;                                     // mulprojection S3:1715
+    if (nrm == 0.0f) pr = 0;                                // S3:1720-1724
+    if (a.scatter_f) ad = ad + rebin_sum(g, a.scatter_f, fplane, v, col);  // addprojection S3:1753
+    ad = ad * at;                                           // mulprojection S3:1769
+    float inv;                                              // norm_ANW S3:809-817
+    const float am = at < 1.0f ? 1.0f : at;
+    if (nrm > 0.0f) {
+        const float den = nrm * am;
+        inv = den > g.eps2 ? 1.0f / den : 0.0f;
+    } else {
+        inv = nrm;
+    }
+    o_pr = pr;
+    o1 = ad;
+    o_inv = inv;
+}
+
+// Staging of internal planes [p_lo, p_hi): a workgroup computes a tile of 32 planes x 64 bins of
+// one view in the files' layout (bins contiguous, coalesced reads), turns it in LDS, and writes
+// it in the pair-column layout (planes contiguous, coalesced writes).
+__global__ void __launch_bounds__(256) stage_kernel(const DevGeom g, StageArgs a) {
+    __shared__ float t_o1[64][33], t_inv[64][33];
+    __shared__ int16_t t_pr[64][34];
+    const int N = g.N, V = g.views;
+    const int p0 = a.p_lo + 32 * (int)blockIdx.x, v = blockIdx.y, c0 = 64 * (int)blockIdx.z;
+    for (int e = threadIdx.x; e < 32 * 64; e += blockDim.x) {
+        const int pl = e >> 6, cl = e & 63, p = p0 + pl, col = c0 + cl;
+        if (p >= a.p_hi || col >= N) continue;
+        float o1 = 0.0f, inv = 0.0f;
+        int16_t pr = 0;
+        stage_one(g, a, p, v, col, o1, inv, pr);
+        t_o1[cl][pl] = o1;
+        t_inv[cl][pl] = inv;
+        t_pr[cl][pl] = pr;
+    }
+    __syncthreads();
+    const int pair = v % (V / 2), h = v / (V / 2);
+    for (int e = threadIdx.x; e < 32 * 64; e += blockDim.x) {
+        const int cl = e >> 5, pl = e & 31, p = p0 + pl, col = c0 + cl;
+        if (p >= a.p_hi || col >= N) continue;
+        const int th = a.plane_seg[p];
+        const size_t o = ((size_t)(pair * 2 + h) * N + col) * g.spp + g.seg_data[th] + (p - g.segment_offset[th]);
+        if (a.weighting < 3) {
+            a.data[o] = t_o1[cl][pl];
+        } else {
+            a.prompt[o] = t_pr[cl][pl];
+            a.add[o] = t_o1[cl][pl];
+        }
+        a.invnorm[o] = t_inv[cl][pl];
     }
 }
 
@@ -1671,9 +1700,8 @@ static void stage_launch(const DevGeom &g, StageArgs a, int th0, int nth, int p_
     a.p_hi = p_hi;
     if (a.norm_f && g.x_rebin * g.v_rebin > 1 && nth > 0)
         hipLaunchKernelGGL(gapmask_kernel, dim3(256, nth), dim3(256), 0, s, g, a.norm_f, const_cast<uint8_t *>(a.gapmask), th0);
-    const size_t n = (size_t)(p_hi - p_lo) * g.views * g.N;
-    const int blocks = (int)std::min<size_t>(8192, (n + 255) / 256);
-    if (blocks > 0) hipLaunchKernelGGL(stage_kernel, dim3(blocks), dim3(256), 0, s, g, a);
+    if (p_hi > p_lo)
+        hipLaunchKernelGGL(stage_kernel, dim3((p_hi - p_lo + 31) / 32, g.views, (g.N + 63) / 64), dim3(256), 0, s, g, a);
     if (finish) hipLaunchKernelGGL(nzmask_kernel, dim3(g.views / 2), dim3(256), 0, s, g, a.colmask, g.nzmask);
 }
 
