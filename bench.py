@@ -531,6 +531,7 @@ def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
     keep = sorted(range(nframes), key=lambda f: counts[f])
     keep = ([keep[0], keep[-1]] + keep[1:-1])[:max(fp64_frames, fp32_frames, 1)]
     adds32 = {}
+    half_range = [0, 0]            # FP16 conversions: inputs sent to +-inf, to subnormal or zero
     factors, invnorm = None, None
     for f in range(nframes):
         act = [1.0 + 3.0 * math.exp(-r * mids[f]) for r in rates]
@@ -541,7 +542,11 @@ def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
         factors = (st["norm"], st["atten"])
         pr, ad, inv = g.stage(st["prompt"], st["delayed"], st["norm"], st["scatter"], st["atten"])
         prompts[f].copy_(pr)
-        adds16[f].copy_(g.to_half(ad))
+        a16, (o_, u_) = g.to_half_range(ad)
+        adds16[f].copy_(a16)
+        half_range[0] += o_
+        half_range[1] += u_
+        del a16
         if f in keep:
             adds32[f] = ad
         if invnorm is None:
@@ -552,7 +557,9 @@ def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
             log(f"c5: {f + 1}/{nframes} frames generated and staged")
     del factors
     torch.cuda.synchronize()
-    inv16 = g.to_half(invnorm)
+    inv16, (o_, u_) = g.to_half_range(invnorm)
+    log(f"c5: FP16 additive terms: {half_range[0]} overflowed, {half_range[1]} subnormal or zero; "
+        f"1/(N*A): {o_} overflowed, {u_} subnormal or zero")
     nf16 = g.calculate_normfac_mixed(inv16)
     nf32 = g.calculate_normfac(invnorm)
     stream = torch.cuda.Stream()
@@ -578,6 +585,8 @@ def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
     run_batched(imgs, True, prompts, adds16, iterations)
     t16 = time.perf_counter() - t0
     log(f"c5: fp16 batched {nframes} frames x {g.subsets} subsets x {iterations} iterations in {t16:.3f} s")
+    out["fp16_inputs_overflowed"] = half_range[0] + o_
+    out["fp16_inputs_subnormal_or_zero"] = {"additive": half_range[1], "invnorm": u_}
     out["fp16_batched_frames_per_s"] = round(nframes / t16, 3)
     # FP32 batched form on fp32_frames frames (the same work per frame)
     sel = keep[:fp32_frames]
@@ -670,6 +679,9 @@ def main():
     ap.add_argument("--c5-iterations", type=int, default=3)
     ap.add_argument("--c5-fp32-frames", type=int, default=4,
                     help="frames of that study also run in the FP32 batched and single-frame forms")
+    ap.add_argument("--precision", choices=("fp32", "fp16"), default="fp32",
+                    help="timed subset path: fp32 (the reference's arithmetic, bit-exact) or fp16 (the "
+                         "precision form: FP16 additive term); the reported line is fp32 by default")
     ap.add_argument("--no-host-stage", dest="host_stage", action="store_false",
                     help="skip the staging from host memory")
     ap.add_argument("--c1-iterations", type=int, default=5,
@@ -751,6 +763,16 @@ def main():
     image = g.init_image()
     image0 = image.clone()
     order = [int(s) for s in g.order]
+    fp16 = args.precision == "fp16"
+    if fp16:
+        # the precision form on the same workload: FP16 additive term, FP32 arithmetic
+        add16 = g.to_half(add)
+
+        def subset(i):
+            g.osem_subset_mixed(image, normfac, prompt, add16, i)
+    else:
+        def subset(i):
+            g.osem_subset(image, normfac, prompt, add, i)
     shape, tune, bp_choice = None, {}, None
     if not args.no_tune:
         from hrrt_reconstruction_amd.autotune import tune_backproject, tune_forward
@@ -759,14 +781,14 @@ def main():
         bp_choice, tune_bp = tune_backproject(g, image, normfac, prompt, add)
         log("autotune back " + ", ".join(f"{k}: {v:.3f} ms" for k, v in tune_bp.items()) + f" -> {bp_choice}")
     for i in range(args.warmup):
-        g.osem_subset(image, normfac, prompt, add, order[i % g.subsets])
+        subset(order[i % g.subsets])
     torch.cuda.synchronize()
 
     g.timing_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        g.osem_subset(image, normfac, prompt, add, order[(args.warmup + i) % g.subsets])
+        subset(order[(args.warmup + i) % g.subsets])
         if (args.warmup + i) % g.subsets == g.subsets - 1:
             g.end_iteration(image)
     torch.cuda.synchronize()
@@ -838,7 +860,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if not fp16 else "f32 (FP16 additive term)",
         "data": "synthetic",
         "config": {"workload": "HRRT span 3, 256x256x207 image, 6367-plane 256x288 sinogram, 200M prompts, "
                                "randoms 10%, scatter 15%, 16 subsets (9 view pairs per subset)",

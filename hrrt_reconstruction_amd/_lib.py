@@ -99,6 +99,7 @@ SIGNATURES = {
     "hrrt_osem_subset_frames": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _I, _VP]),
     "hrrt_osem_iteration_frames": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP]),
     "hrrt_to_half": (_I, [_VP, _VP, C.c_longlong, _VP]),
+    "hrrt_to_half_range": (_I, [_VP, _VP, C.c_longlong, _VP, _VP]),
     "hrrt_calculate_normfac_mixed": (_I, [_VP, _VP, _VP, _VP]),
     "hrrt_osem_subset_mixed": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
     "hrrt_osem_subset_frames_mixed": (_I, [_VP, _I, _VP, _VP, _VP, _VP, _VP, _I, _VP, _VP]),

@@ -120,6 +120,11 @@ struct hrrt_ctx {
     }
     void mark(int id, hipStream_t s) {
         if (!timing) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;   // eager launches only
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+            (void)hipGetLastError();
+            return;
+        }
         size_t i = ev_used;
         hipEvent_t e = next_event();
         if (e) {
@@ -1012,6 +1017,7 @@ static int frames_subset_launch(hrrt_ctx *ctx, int nframes, float *d_images, con
         io.invnorm = pr.invnorm16;
         io.ll_part = ctx->ll_part;
     }
+    ctx->mark(0, s);
     HIP_TRY(hrrt::launch_forward(ctx->dg, b, d_images, ctx->f_ratio, io, pr.mixed ? hrrt::FP_RATIO16 : hrrt::FP_RATIO, s));
     if (pr.mixed && pr.loglik) {
         int maxt, nsplit, threads, items;
@@ -1020,8 +1026,11 @@ static int frames_subset_launch(hrrt_ctx *ctx, int nframes, float *d_images, con
     }
     // the subset update back-projects under the schedule in both forms (incremental by default);
     // the precision form sums the view pairs of the rotation with compensation
+    ctx->mark(1, s);
     HIP_TRY(backproject_scheduled(ctx, b, ctx->f_ratio, 0, 0, ctx->fA, ctx->fB, s));
+    ctx->mark(2, s);
     HIP_TRY(hrrt::launch_rotate_update(ctx->dg, b, ctx->fA, ctx->fB, d_images, d_normfac_subset, pr.mixed, s));
+    ctx->mark(-1, s);
     return HRRT_OK;
 }
 
@@ -1111,6 +1120,13 @@ extern "C" int hrrt_osem_iteration_frames(hrrt_ctx *ctx, int nframes, float *d_i
 extern "C" int hrrt_to_half(const float *d_in, uint16_t *d_out, long long n, void *stream) {
     if (!d_in || !d_out || n < 0) return fail(HRRT_ERR_ARG, "invalid argument");
     HIP_TRY(hrrt::launch_to_half(d_in, d_out, (size_t)n, S(stream)));
+    return HRRT_OK;
+}
+
+extern "C" int hrrt_to_half_range(const float *d_in, uint16_t *d_out, long long n, unsigned long long *d_counts,
+                                  void *stream) {
+    if (!d_in || !d_out || !d_counts || n < 0) return fail(HRRT_ERR_ARG, "invalid argument");
+    HIP_TRY(hrrt::launch_to_half_range(d_in, d_out, (size_t)n, d_counts, S(stream)));
     return HRRT_OK;
 }
 

@@ -89,6 +89,9 @@ hipError_t launch_rotate_sens(const DevGeom &g, const PairBatch &b, const float 
 // per-frame sums of the forward projector's log-likelihood partials (fixed order, deterministic)
 hipError_t launch_ll_reduce(const double *part, int per_frame, int nframes, double *out, hipStream_t s);
 hipError_t launch_to_half(const float *in, uint16_t *out, size_t n, hipStream_t s);
+// the same, counting inputs that overflow to +-inf ([0]) and non-zero inputs that become FP16
+// subnormals or zero ([1]) into counts (cleared first by a kernel, so it may be captured)
+hipError_t launch_to_half_range(const float *in, uint16_t *out, size_t n, unsigned long long *counts, hipStream_t s);
 // FP64 sums over a pair-column sinogram: [0] sum prompt, [1] sum w*add, [2] sum w*est, w = 1/(N*A)
 hipError_t launch_count_moments(const DevGeom &g, const int16_t *prompt, const float *add, const float *invnorm,
                                 const float *est, int view0, int view_stride, int npairs, double *part,

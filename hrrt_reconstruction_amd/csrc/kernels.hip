@@ -1174,6 +1174,10 @@ __global__ void fill_kernel(float4 *__restrict__ p, size_t n4) {
         p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+__global__ void fill_u64_kernel(unsigned long long *__restrict__ p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Staging (prepare_osem3dw3 S3:1666-1770, prepare_norm S3:841-1003 with
 // norm_ANW S3:796-821, read_flat_* FIO:110-259, sino_rebin FIO:29-74).
@@ -1442,6 +1446,30 @@ __global__ void to_half_kernel(const float *__restrict__ in, uint16_t *__restric
         out[i] = __half_as_ushort(__float2half_rn(in[i]));
 }
 
+// The same conversion, counting the finite inputs that become +-inf ([0]) and the non-zero
+// inputs that become FP16 subnormals or zero ([1]); one atomic per wave and counter.
+__global__ void to_half_range_kernel(const float *__restrict__ in, uint16_t *__restrict__ out, size_t n,
+                                     unsigned long long *__restrict__ counts) {
+    unsigned long long over = 0, small = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float x = in[i];
+        const __half h = __float2half_rn(x);
+        const uint16_t b = __half_as_ushort(h);
+        out[i] = b;
+        const bool finite_in = isfinite(x);
+        over += (finite_in && (b & 0x7fffu) == 0x7c00u) ? 1 : 0;   // exponent all ones, mantissa 0
+        small += (x != 0.0f && finite_in && (b & 0x7c00u) == 0) ? 1 : 0;   // exponent 0: subnormal or zero
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        over += __shfl_xor(over, o);
+        small += __shfl_xor(small, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (over) atomicAdd(&counts[0], over);
+        if (small) atomicAdd(&counts[1], small);
+    }
+}
+
 // moments of one pair-column block (pair p, half h, column col): sum prompt, sum w*add, sum w*est
 __global__ void __launch_bounds__(256) count_moments_kernel(const DevGeom g, const int16_t *__restrict__ prompt,
                                                             const float *__restrict__ add,
@@ -1596,6 +1624,12 @@ hipError_t launch_ll_reduce(const double *part, int per_frame, int nframes, doub
 
 hipError_t launch_to_half(const float *in, uint16_t *out, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(to_half_kernel, dim3(2048), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_half_range(const float *in, uint16_t *out, size_t n, unsigned long long *counts, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(64), 0, s, counts, 2);
+    hipLaunchKernelGGL(to_half_range_kernel, dim3(2048), dim3(256), 0, s, in, out, n, counts);
     return hipGetLastError();
 }
 

@@ -370,6 +370,18 @@ class Osem3D(Layout):
         _lib.call("hrrt_to_half", _p(t), _p(out), int(t.numel()), _stream())
         return out
 
+    @staticmethod
+    def to_half_range(t):
+        """to_half, also returning (overflowed, underflowed): the finite inputs that became +-inf
+        and the non-zero inputs that became FP16 subnormals or zero."""
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32 or not t.is_contiguous():
+            raise TypeError("to_half_range takes a contiguous float32 CUDA tensor")
+        out = torch.empty(t.shape, dtype=torch.int16, device=t.device)
+        counts = torch.empty(2, dtype=torch.int64, device=t.device)
+        _lib.call("hrrt_to_half_range", _p(t), _p(out), int(t.numel()), _p(counts), _stream())
+        c = counts.cpu()
+        return out, (int(c[0]), int(c[1]))
+
     def calculate_normfac_mixed(self, invnorm16):
         src = self._pc(invnorm16, torch.int16, "invnorm16")
         nf = torch.zeros((self.subsets, self.N, self.N, self.Z4), dtype=torch.float32, device=self.device)

@@ -221,6 +221,12 @@ int hrrt_osem_iteration_frames(hrrt_ctx *ctx, int nframes, float *d_images, cons
  * moments.  Results are within tolerance of an FP64 projector, not bit-identical to
  * the FP32 path. */
 int hrrt_to_half(const float *d_in, uint16_t *d_out, long long n, void *stream);
+/* hrrt_to_half that also reports the range of the input (no counterpart in the reference):
+ * d_counts[0] = finite inputs that become +-inf (|x| >= 65520), d_counts[1] = non-zero
+ * finite inputs that become FP16 subnormals or zero (|x| < 2^-14); d_counts is two
+ * uint64 in device memory, cleared by the call. */
+int hrrt_to_half_range(const float *d_in, uint16_t *d_out, long long n, unsigned long long *d_counts,
+                       void *stream);
 /* CalculateNormfac (S3:3760) from FP16 1/(N*A), Kahan accumulation. */
 int hrrt_calculate_normfac_mixed(hrrt_ctx *ctx, const uint16_t *d_invnorm16, float *d_normfac, void *stream);
 /* hrrt_osem_subset with FP16 additive term.  When d_loglik is not NULL, *d_loglik

@@ -47,6 +47,13 @@ def test_to_half_rounds_like_ieee(torch_dev, tiny):
                         6.1e-5, 5.9e-8, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11]]).astype(np.float32)
     got = from_dev(g.to_half(to_dev(x)))
     assert np.array_equal(got, half_bits(x))
+    # the range report counts what numpy's own conversion sends to +-inf and to subnormal/zero
+    got2, (over, under) = g.to_half_range(to_dev(x))
+    assert np.array_equal(from_dev(got2), got)
+    h = half_bits(x).view(np.float16).astype(np.float32)
+    assert over == int(np.sum(np.isinf(h) & np.isfinite(x)))
+    assert under == int(np.sum((x != 0) & (np.abs(h) < 2.0 ** -14)))
+    assert over == 2 and under >= 4096          # +-7e4, and every |x| <= 1e-6
 
 
 def test_forward_vs_fp64(torch_dev, tiny):
