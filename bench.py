@@ -395,7 +395,7 @@ def listmode_leg(torch, events, steps, seed, cpu_events, span):
     log(f"list mode: {events} events in {ms:.2f} ms ({events / ms / 1e6:.2f} G events/s)")
     res = {"events": events, "words": int(words.size), "ms_per_histogram": round(ms, 3),
            "events_per_s": round(events / (ms * 1e-3), 1), "prompts_binned": st["prompts"], "span": span,
-           "note": "whole stream per call: zeroing, chunk walk, histogram, int16 finalize, statistics", "cpu_baseline": None}
+           "note": "whole stream per call: chunk walk, event keys, radix sort by sinogram tile, per-tile counts into int16 bins, statistics", "cpu_baseline": None}
     del sino, cp, cd, d
     torch.cuda.empty_cache()
     if cpu_events:

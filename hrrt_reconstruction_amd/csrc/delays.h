@@ -72,8 +72,17 @@ struct hrrt_delays {
     // list-mode histogramming: the LUT by (module pair, A crystal, B crystal) on the device,
     // and its scratch (uploaded / allocated on first use)
     int4 *d_sol = nullptr;                   // [21][144][144] {nsino, bits(d), bits(z), 0}
-    int *d_lm_counts = nullptr;              // [2][nplanes*nsino] prompt (or net) and delayed counts
-    unsigned *d_lm_chunks = nullptr;         // per 2048-word chunk: walk summaries, entry parity
+    unsigned *d_lm_keys = nullptr;           // [2][lm_key_cap] key slots of the events (2*bin + delayed), then sorted
+    long long lm_key_cap = 0;
+    void *d_lm_sort_tmp = nullptr;           // radix sort scratch
+    size_t lm_sort_bytes = 0;
+    unsigned *d_lm_tiles = nullptr;          // [ntiles + 1] first sorted key of each sinogram tile
+    unsigned short *d_lm_coin_seg = nullptr; // [lm_coin_cap][8 heads][1024] crystal entries of a chunk's events
+    unsigned *d_lm_coin_cnt = nullptr;       // [lm_coin_cap][8 heads] entries per chunk and head
+    long long lm_coin_cap = 0;
+    unsigned *d_lm_coin_part = nullptr;      // per chunk range and head: [2 maps][2 layers][104][72] counts
+    unsigned *d_lm_chunks = nullptr;         // [65][lm_chunk_cap]: per 2048-word chunk its walk summary and
+                                             // entry parity, then its 64 out-of-sync mask words
     long long lm_chunk_cap = 0;
     unsigned long long *d_lm_stats = nullptr;
 

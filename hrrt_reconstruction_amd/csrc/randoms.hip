@@ -434,7 +434,8 @@ extern "C" int hrrt_delays_create(const hrrt_delays_params *params, hrrt_delays 
 extern "C" void hrrt_delays_destroy(hrrt_delays *c) {
     if (!c) return;
     for (void *p : {(void *)c->d_lor, (void *)c->d_lor_off, (void *)c->d_seg, (void *)c->d_z, (void *)c->d_sings,
-                    (void *)c->d_cols, (void *)c->d_sol, (void *)c->d_lm_counts, (void *)c->d_lm_chunks,
+                    (void *)c->d_cols, (void *)c->d_sol, (void *)c->d_lm_keys, (void *)c->d_lm_sort_tmp, (void *)c->d_lm_tiles,
+                    (void *)c->d_lm_coin_seg, (void *)c->d_lm_coin_cnt, (void *)c->d_lm_coin_part, (void *)c->d_lm_chunks,
                     (void *)c->d_lm_stats})
         if (p) (void)hipFree(p);
     delete c;

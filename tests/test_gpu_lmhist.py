@@ -1,6 +1,7 @@
 """List-mode histogramming on the GPU against the oracle, bit for bit.
 
-Histograms are integer sums (atomics on the GPU, a sequential walk in the oracle), so every
+Histograms are integer counts (per-tile counts of radix-sorted event keys on the GPU, a
+sequential walk in the oracle), so every
 sinogram bin, delayed byte, coincidence-map count and statistic must be identical.  Streams
 carry stray out-of-sync words, rejected, transmission and DOI 2-6 events, block-singles
 tags, and span more than one 2^21-word packet.
@@ -145,9 +146,9 @@ def test_list_mode_to_osem_subset(torch_dev, oracle_lib):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-def test_unaligned_outputs_take_the_scalar_finalize(torch_dev, stream, mode):
-    """Outputs that are not 16-byte (sinogram) or 8-byte (delayed bytes) aligned go through the
-    one-bin-per-thread finalize; the bins must equal the aligned call's, which the oracle pins."""
+def test_unaligned_outputs(torch_dev, stream, mode):
+    """Outputs that are not 16-byte (sinogram) or 8-byte (delayed bytes) aligned: the bins must
+    equal the aligned call's, which the oracle pins, and no byte outside them is written."""
     import ctypes as C
     from hrrt_reconstruction_amd import _lib, lmhist
     from hrrt_reconstruction_amd.randoms import GenDelays
@@ -170,3 +171,18 @@ def test_unaligned_outputs_take_the_scalar_finalize(torch_dev, stream, mode):
     if mode == 1:
         assert torch_dev.equal(delayed, ref_d.reshape(-1))
         assert torch_dev.all(dbuf[:3] == 7) and torch_dev.all(dbuf[3 + n:] == 7)
+
+
+@pytest.mark.parametrize("nwords", [0, 2, 4097, 2048 * 5 + 6])
+def test_short_and_ragged_streams(torch_dev, stream, nwords):
+    """Streams shorter than a chunk, ending inside one, and empty: every key slot past the
+    events stays unused, and the sinogram equals the oracle's (all zero when empty)."""
+    from hrrt_reconstruction_amd.randoms import GenDelays
+    print(f"lmhist short stream nwords={nwords}", flush=True)
+    g, D = GenDelays(span=9), O.Delays(span=9)
+    words = np.ascontiguousarray(stream[:nwords])
+    d_words = torch_dev.from_numpy(words.view(np.int32)).cuda()
+    for mode in (0, 1):
+        st = _check(torch_dev, g, D, words, d_words, mode=mode)
+        if nwords == 0:
+            assert st["events"] == 0
