@@ -23,7 +23,8 @@
 //   lm_coin_kernel     per head and range of chunks: the head's crystal entries counted in LDS
 //   lm_coin_sum_kernel the ranges' counts summed into the coincidence maps
 // No global atomics touch a sinogram bin or a crystal counter: scattered device-scope atomics
-// over the 1.9 GB count array and the 0.5 MB maps cost 5.6 ms of 6.8 for 40 M events here.
+// into a 1.9 GB int32 count array and the 0.5 MB maps took 5.6 ms of a 6.8 ms call for 40 M
+// events at span 3, against 1.9 ms for the whole call this way.
 // Integer counts do not depend on the order of the events, so the result equals the
 // reference's sequential histogram exactly.
 #include <hip/hip_runtime.h>
@@ -128,7 +129,7 @@ __device__ unsigned chunk_mask_word(const unsigned *__restrict__ w, long long cs
     const long long pend = packet_end(cs, nwords);
     const int par = lane >> 5, mi = lane & 3, kk = (lane & 31) >> 2;
     unsigned mine = 0;
-#pragma unroll 2
+#pragma unroll
     for (int k = 0; k < CHUNK / 256; k++) {
         const long long q0 = cs + 256 * k + 4 * lane;
         unsigned v[5];
@@ -186,12 +187,14 @@ __global__ __launch_bounds__(WAVES * 64) void lm_summary_kernel(LmArgs a, long l
     masks[wv][lane] = m;
     a.cmask[c * 64 + lane] = m;
     __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
+    // the walks from entry 0 (lane 0) and entry 1 (lane 32) side by side
+    int e = 0;
+    if ((lane & 31) == 0) {
         int n;
-        const int e0 = chunk_walk(masks[wv], 0, nullptr, n);
-        const int e1 = chunk_walk(masks[wv], 1, nullptr, n);
-        a.chunks[c] = (unsigned)(e0 | (e1 << 1));
+        e = chunk_walk(masks[wv], lane >> 5, nullptr, n);
     }
+    const int e1 = __shfl(e, 32);
+    if (lane == 0) a.chunks[c] = (unsigned)(e | (e1 << 1));
 }
 
 // one workgroup per packet: the chunk summaries staged in LDS, walked in order by one thread
@@ -353,15 +356,14 @@ __global__ __launch_bounds__(WAVES * 64) void lm_hist_kernel(LmArgs a, long long
                      !(a.mode == 2 && v.type != 0);
         return v;
     };
-    auto step = [&](int i, unsigned ew1, unsigned ew2, const Ev &v, int4 e) __attribute__((always_inline)) {
-        const long long q = cs + i * 64 + lane;
+    auto step = [&](long long q, unsigned ew1, unsigned ew2, const Ev &v, int4 e) __attribute__((always_inline)) {
         // tags (process_tagword histogram_mp.cpp:225-295)
         const bool is_tag = v.type == 2;
         const unsigned tag = (ew1 & 0xffffu) | ((ew2 & 0xffffu) << 16);
         const bool time_tag = is_tag && (tag & 0xE0000000u) == 0x80000000u;
         n_tags += is_tag;
         n_time += time_tag;
-        if (time_tag) last_time = ((unsigned long long)q << 30) | (tag & 0x3fffffffu);   // q grows with i
+        if (time_tag) last_time = ((unsigned long long)q << 30) | (tag & 0x3fffffffu);   // q grows along the lane
         if (is_tag && (tag & 0xE0000000u) == 0xA0000000u) {
             const int block = (int)((tag & 0x1ff80000u) >> 19);
             const unsigned val = tag & 0x0007ffffu;
@@ -408,29 +410,45 @@ __global__ __launch_bounds__(WAVES * 64) void lm_hist_kernel(LmArgs a, long long
         }
         nk += (unsigned)__popcll(bal);
     };
-    // the word pairs in groups of G rows of 64: the next group's words are loaded and this
-    // group's LUT entries gathered before any of the group is processed
-    constexpr int G = 4, NIT = CHUNK / 64;
+    // The walk visits one position of each aligned word pair (2k, 2k + 1): the one of its
+    // parity, entry parity flipped by every out-of-sync word before 2k; an out-of-sync word is
+    // visited but not decoded, and when it is the pair's even word the walk goes on to the odd
+    // one.  Each lane takes a pair per row; rows of 64 pairs in groups of G: the next group's
+    // words are loaded and this group's LUT entries gathered before any of the group is processed.
+    constexpr int G = 4, NIT = CHUNK / 128;
     static_assert(NIT % G == 0, "whole groups");
-    auto load_words = [&](int i0, unsigned *x1, unsigned *x2, bool *l) __attribute__((always_inline)) {
+    const unsigned short *sw = sws[wv];
+    auto load_words = [&](int i0, unsigned *x1, unsigned *x2, int *pp, bool *l) __attribute__((always_inline)) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            const int r = (i0 + g) * 64 + lane;
-            const long long q = cs + r;
-            l[g] = q + 1 < pend && q >= lo && q < hi && visited(r, entry, sws[wv], nsw);
+            const int x = 2 * ((i0 + g) * 64 + lane);
+            int n = 0, m = nsw;   // out-of-sync words visited before x
+            while (n < m) {
+                const int mid = (n + m) >> 1;
+                if (sw[mid] < x) n = mid + 1;
+                else m = mid;
+            }
+            const int par = (entry + n) & 1;
+            int p = x + par;
+            if (n < nsw && sw[n] == p) p = (par == 0 && !(n + 1 < nsw && sw[n + 1] == x + 1)) ? x + 1 : -1;
+            const long long q = cs + p;
+            pp[g] = p;
+            l[g] = p >= 0 && q + 1 < pend && q >= lo && q < hi;
             x1[g] = l[g] ? a.w[q] : 0u;
             x2[g] = l[g] ? a.w[q + 1] : 0u;
         }
     };
     unsigned w1[G], w2[G];
+    int pos[G];
     bool lv[G];
-    load_words(0, w1, w2, lv);
+    load_words(0, w1, w2, pos, lv);
     for (int i0 = 0; i0 < NIT; i0 += G) {
         unsigned n1[G], n2[G];
+        int np[G];
         bool nl[G];
 #pragma unroll
-        for (int g = 0; g < G; g++) n1[g] = n2[g] = 0u, nl[g] = false;
-        if (i0 + G < NIT) load_words(i0 + G, n1, n2, nl);
+        for (int g = 0; g < G; g++) n1[g] = n2[g] = 0u, np[g] = -1, nl[g] = false;
+        if (i0 + G < NIT) load_words(i0 + G, n1, n2, np, nl);
         Ev v[G];
         int4 e[G];
 #pragma unroll
@@ -441,9 +459,9 @@ __global__ __launch_bounds__(WAVES * 64) void lm_hist_kernel(LmArgs a, long long
                 e[g] = a.sol[((v[g].mpe * NXD) + v[g].ax + NX * v[g].doia) * NXD + v[g].bx + NX * v[g].doib];
         }
 #pragma unroll
-        for (int g = 0; g < G; g++) step(i0 + g, w1[g], w2[g], v[g], e[g]);
+        for (int g = 0; g < G; g++) step(cs + pos[g], w1[g], w2[g], v[g], e[g]);
 #pragma unroll
-        for (int g = 0; g < G; g++) w1[g] = n1[g], w2[g] = n2[g], lv[g] = nl[g];
+        for (int g = 0; g < G; g++) w1[g] = n1[g], w2[g] = n2[g], pos[g] = np[g], lv[g] = nl[g];
     }
     no_keys_from(nk);
     if (a.cseg) {
@@ -678,7 +696,8 @@ extern "C" int hrrt_lm_histogram(hrrt_delays *c, const uint32_t *d_words, long l
     if (opt->mode < 0 || opt->mode > 2) return hrrt_dl::fail(HRRT_ERR_ARG, "mode must be 0, 1 or 2");
     if (nwords >= (1ll << 34)) return hrrt_dl::fail(HRRT_ERR_ARG, "stream longer than 2^34 words");
     if ((long long)c->nplanes * c->nsino > (1ll << 30)) return hrrt_dl::fail(HRRT_ERR_ARG, "sinogram larger than 2^30 bins");
-    if (nwords / 2 > 0xffffffffll) return hrrt_dl::fail(HRRT_ERR_ARG, "stream longer than 2^33 words");
+    // key slots (one per two words) are indexed in 32 bits
+    if (nwords > (1ll << 33) - (1ll << 22)) return hrrt_dl::fail(HRRT_ERR_ARG, "stream longer than 2^33 - 2^22 words");
     hipStream_t s = (hipStream_t)stream;
     const long long nchunks = (nwords + CHUNK - 1) / CHUNK;
     const bool coin = d_coin_p || d_coin_d;
@@ -734,7 +753,7 @@ extern "C" int hrrt_lm_histogram(hrrt_delays *c, const uint32_t *d_words, long l
         DTRY(rocprim::radix_sort_keys(c->d_lm_sort_tmp, bytes, c->d_lm_keys, sorted, (size_t)nk, TILE_BITS + 1, end_bit, s));
     }
     lm_tiles_kernel<<<(unsigned)((ntiles + 1 + 255) / 256), 256, 0, s>>>(sorted, nk, ntiles, c->d_lm_tiles);
-    const unsigned tgrid = (unsigned)std::min(ntiles, 2048ll);
+    const unsigned tgrid = (unsigned)std::min(ntiles, 1024ll);   // every workgroup resident (4 per CU)
     const bool both = opt->mode == 1;
     const int vec = (uintptr_t)d_sino % 16 == 0 &&
                     (!both || (d_delayed ? (uintptr_t)d_delayed % 16 == 0 : S % 8 == 0));

@@ -59,7 +59,9 @@ typedef struct {
  * delayed coincidence maps (either may be NULL).  Outputs are overwritten.  stats (optional,
  * host) makes the call synchronous.  Mode 7 (transmission) returns HRRT_ERR_UNSUPPORTED.
  * The context keeps the call's scratch: two 4-byte key slots per two stream words (the binned
- * events, then sorted by sinogram tile) and the radix sort's storage; sinograms up to 2^30 bins. */
+ * events, then sorted by sinogram tile) and the radix sort's storage; with coincidence maps also
+ * 8 bytes per word of per-head crystal entries.  Streams up to 2^33 - 2^22 words, sinograms up
+ * to 2^30 bins. */
 int hrrt_lm_histogram(hrrt_delays *ctx, const uint32_t *d_words, long long nwords, const hrrt_lm_options *opt,
                       int16_t *d_sino, uint8_t *d_delayed, uint32_t *d_coin_p, uint32_t *d_coin_d,
                       hrrt_lm_stats *stats, void *stream);
