@@ -104,7 +104,8 @@ __device__ inline float lane_value(float v, int lane) {
 
 template <bool DIRECT>
 __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, const float *__restrict__ sings,
-                                                                    float tauftime, float *__restrict__ cols) {
+                                                                    float tauftime, float *__restrict__ cols,
+                                                                    int cstride) {
     extern __shared__ unsigned long long col[];
     // segment entry by e = 2*(int)seg + (seg < 0) + 64: {first plane | last plane << 16, file offset}
     __shared__ int2 seg[128];
@@ -197,26 +198,45 @@ store:
     const double down = ldexp(1.0, -kexp);
     if (DIRECT) {   // plane-major output [nplanes][nsino]
         for (int i = tid; i < t.nplanes; i += COL_THREADS) cols[(size_t)i * t.nsino + bin] = (float)((double)(long long)col[i] * down);
-    } else {        // bin-major columns [nsino][nplanes]
-        float *dst = cols + (size_t)bin * t.nplanes;
+    } else {        // bin-major columns [nsino][cstride], cstride a multiple of 64 planes
+        float *dst = cols + (size_t)bin * cstride;
         for (int i = tid; i < t.nplanes; i += COL_THREADS) dst[i] = (float)((double)(long long)col[i] * down);
     }
 }
 
-// [nsino][nplanes] -> [nplanes][nsino]; 64 x 64 tiles through LDS, 256 threads (4 rows each)
+// [nsino][cstride] columns -> [nplanes][nsino]; 64 x 64 tiles through LDS, 256 threads (16
+// elements each), TT tiles along the planes per workgroup with the next tile's loads in flight
+// while the current one is stored.  A column stride of whole 256-byte lines keeps every tile
+// row's read on line boundaries.
+constexpr int TT = 4;
+
 __global__ __launch_bounds__(256) void column_transpose_kernel(const float *__restrict__ cols, float *__restrict__ out,
-                                                               int nsino, int nplanes) {
-    __shared__ float tile[64][65];
-    const int p0 = blockIdx.x * 64, b0 = blockIdx.y * 64;
+                                                               int nsino, int nplanes, int cstride) {
+    __shared__ float tile[2][64][65];
+    const int b0 = blockIdx.y * 64, pt0 = blockIdx.x * TT * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int r = ty; r < 64; r += 4) {
-        const int b = b0 + r, p = p0 + tx;
-        tile[r][tx] = (b < nsino && p < nplanes) ? cols[(size_t)b * nplanes + p] : 0.0f;
-    }
-    __syncthreads();
-    for (int r = ty; r < 64; r += 4) {
-        const int p = p0 + r, b = b0 + tx;
-        if (p < nplanes && b < nsino) out[(size_t)p * nsino + b] = tile[tx][r];
+    float v[16];
+    auto load = [&](int p0) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int b = b0 + ty + 4 * k, p = p0 + tx;
+            v[k] = (b < nsino && p < nplanes) ? cols[(size_t)b * cstride + p] : 0.0f;
+        }
+    };
+    load(pt0);
+    for (int i = 0; i < TT; i++) {
+        const int p0 = pt0 + 64 * i;
+        if (p0 >= nplanes) break;
+        float(*T)[65] = tile[i & 1];
+#pragma unroll
+        for (int k = 0; k < 16; k++) T[ty + 4 * k][tx] = v[k];
+        __syncthreads();
+        if (i + 1 < TT && p0 + 64 < nplanes) load(p0 + 64);
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int p = p0 + ty + 4 * k, b = b0 + tx;
+            if (p < nplanes && b < nsino) out[(size_t)p * nsino + b] = T[tx][ty + 4 * k];
+        }
     }
 }
 
@@ -538,15 +558,16 @@ extern "C" int hrrt_delays_smooth(hrrt_delays *c, const float *d_singles, float 
     sings_prepare_kernel<<<1, PREP_THREADS, 0, s>>>(d_singles, c->d_sings);
     DTRY(hipGetLastError());
     if (c->nplanes <= DIRECT_PLANES) {
-        delays_column_kernel<true><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, d_delays);
+        delays_column_kernel<true><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, d_delays, 0);
         DTRY(hipGetLastError());
         return HRRT_OK;
     }
-    if (!c->d_cols) DTRY(hipMalloc(&c->d_cols, sizeof(float) * (size_t)c->nsino * c->nplanes));
-    delays_column_kernel<false><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, c->d_cols);
+    const int cstride = (c->nplanes + 63) & ~63;
+    if (!c->d_cols) DTRY(hipMalloc(&c->d_cols, sizeof(float) * (size_t)c->nsino * cstride));
+    delays_column_kernel<false><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, c->d_cols, cstride);
     DTRY(hipGetLastError());
-    column_transpose_kernel<<<dim3((c->nplanes + 63) / 64, (c->nsino + 63) / 64), 256, 0, s>>>(c->d_cols, d_delays,
-                                                                                               c->nsino, c->nplanes);
+    column_transpose_kernel<<<dim3((c->nplanes + 64 * TT - 1) / (64 * TT), (c->nsino + 63) / 64), 256, 0, s>>>(c->d_cols, d_delays,
+                                                                                               c->nsino, c->nplanes, cstride);
     DTRY(hipGetLastError());
     return HRRT_OK;
 }
