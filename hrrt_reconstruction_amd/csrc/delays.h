@@ -68,7 +68,6 @@ struct hrrt_delays {
     float2 *d_z = nullptr;
     float *d_sings = nullptr;                // transposed singles [layer][head][x][ring], then column maxima
     float *d_cols = nullptr;                 // [nsino][nplanes rounded up to 64] columns
-    int lds_bytes = 0;
     // list-mode histogramming: the LUT by (module pair, A crystal, B crystal) on the device,
     // and its scratch (uploaded / allocated on first use)
     int4 *d_sol = nullptr;                   // [21][144][144] {nsino, bits(d), bits(z), 0}

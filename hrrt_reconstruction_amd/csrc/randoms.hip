@@ -73,9 +73,11 @@ __device__ inline int fixed_point_exponent(float m, int nlor) {
 
 // Term -> int64 in units of 2^-k, rounded to nearest: x = v*2^k is exact (a power of two), and
 // so are its split into x = hi*2^32 + lo (hi integral) and lo's rounding below 2^32.
+// (x - hi*2^32 is exact: below 2^32 hi is 0, above it x is a multiple of 2^9 and the difference
+// below 2^32 has at most 23 bits, so one fused multiply-add gives it.)
 __device__ inline unsigned long long to_fixed(float x) {
     const float hi = floorf(x * 0x1p-32f);
-    const float lo = x - hi * 0x1p32f;
+    const float lo = __builtin_fmaf(hi, -0x1p32f, x);
     return ((unsigned long long)(long long)(int)hi << 32) + (unsigned long long)(unsigned)rintf(lo);
 }
 
@@ -97,6 +99,8 @@ constexpr int COL_THREADS = 512;
 // output; longer ones (span 3: 6367) go through bin-major columns and a tiled transpose,
 // since 96 workgroups' worth of them per XCD no longer gather in its L2.
 constexpr int DIRECT_PLANES = 4096;
+// The bin-major column path's LDS column (6400 planes: span 3 at maxrd up to 68 has 6367).
+constexpr int COLUMN_PLANES = 6400;
 
 __device__ inline float lane_value(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -106,15 +110,19 @@ template <bool DIRECT>
 __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, const float *__restrict__ sings,
                                                                     float tauftime, float *__restrict__ cols,
                                                                     int cstride) {
-    extern __shared__ unsigned long long col[];
-    // segment entry by e = 2*(int)seg + (seg < 0) + 64: {first plane | last plane << 16, file offset}
+    // segment entry by e = 2*(int)seg + (seg < 0) + 64: {first plane | last plane << 16, 8 * file offset}
     __shared__ int2 seg[128];
+    // the bin's column of planes (a static array: its adds take their LDS base as an immediate)
+    __shared__ unsigned long long col[DIRECT ? DIRECT_PLANES : COLUMN_PLANES];
     // XCD-aware: workgroups are dealt to the 8 XCDs in turn, so XCD x gets bins
     // [x*nsino/8, (x+1)*nsino/8) in order and its L2 gathers whole lines of the plane-major output
     const int per = gridDim.x >> 3;
     const int bin = (blockIdx.x & 7) * per + (blockIdx.x >> 3), tid = threadIdx.x;
     for (int i = tid; i < t.nplanes; i += COL_THREADS) col[i] = 0ull;
-    if (tid < 128) seg[tid] = t.seg[tid];
+    if (tid < 128) {
+        const int2 e = t.seg[tid];
+        seg[tid] = make_int2(e.x, 8 * e.y);
+    }
     const int k0 = t.lor_off[bin], k1 = t.lor_off[bin + 1];
     float mlor = 0.0f;
     for (int k = k0; k < k1; k++) {
@@ -129,6 +137,7 @@ __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, co
     const int by = (wave & 1) * 64 + lane;
     const bool live = by < NY;
     const int w = t.window;
+    const int ay_lo = by - w, ay_hi = by + w;   // A rings inside the ring window of B ring by
     const float2 z0 = t.z[lane];                                          // A rings 0..63
     const float2 z1 = lane < NY - 64 ? t.z[64 + lane] : make_float2(0.0f, 0.0f);   // A rings 64..103
     const float zb = live ? t.z[by].x : 0.0f;
@@ -153,20 +162,23 @@ __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, co
             // one crystal pair per lane for A ring ay (A singles*tau*T awt, ring z za, plane
             // cay): the predicate, column index and fixed-point value of its add.  The
             // reference skips silent A crystals; their terms are exact zeros here.
-            auto prepare = [&](int ay, float awt, float za, float cay, bool &ok, int &idx, unsigned long long &q) {
-                const int dr = by - ay;
+            auto prepare = [&](int ay, float awt, float za, float cay, bool &ok, int &idx8, unsigned long long &q) {
                 const float dz = zb - za;
                 const int plane = (int)(cay + sz * dz);
                 const float s = 0.5f + dz * sd;
                 const int si = (int)s;
                 // e - 64 = 2*(int)seg + (seg < 0) names the code (2*si for seg >= 0, 1 - 2*si
-                // for seg < 0); e outside [0, 128) and impossible pairs map to no plane
-                const int e = min(max(2 * si + (s < 0.0f ? 1 : 0), -64), 63);
-                const long long ent = *reinterpret_cast<const long long *>(&seg[64 + e]);
+                // for seg < 0); e outside [0, 128) and impossible pairs map to no plane.  The
+                // byte offset 8 (e - 64), clamped to [-512, 504], reads seg[e].
+                const int e8 = min(max((si * 16) | (s < 0.0f ? 8 : 0), -512), 504);
+                const long long ent = *reinterpret_cast<const long long *>(reinterpret_cast<const char *>(seg) + 512 + e8);
                 const int first = (int)(ent & 0xffff), last = (int)((ent >> 16) & 0xffff);
-                ok = live & (dr <= w) & (dr >= -w) & (plane >= first) & (plane <= last);
-                idx = plane + (int)(ent >> 32);
+                ok = live & (ay >= ay_lo) & (ay <= ay_hi) & (plane >= first) & (plane <= last);
+                idx8 = (plane << 3) + (int)(ent >> 32);
                 q = to_fixed(bu * awt);
+            };
+            auto add = [&](int idx8, unsigned long long q) {
+                atomicAdd(reinterpret_cast<unsigned long long *>(reinterpret_cast<char *>(col) + idx8), q);
             };
             // A rings row, row+4, ..., two per step: 16 below ring 64 and 10 from it (row < 4)
 #pragma unroll 2
@@ -177,8 +189,8 @@ __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, co
                 unsigned long long qA, qB;
                 prepare(ay, lane_value(a0, ay), lane_value(z0.x, ay), lane_value(z0.y, ay), okA, iA, qA);
                 prepare(ay + 4, lane_value(a0, ay + 4), lane_value(z0.x, ay + 4), lane_value(z0.y, ay + 4), okB, iB, qB);
-                if (okA) atomicAdd(&col[iA], qA);
-                if (okB) atomicAdd(&col[iB], qB);
+                if (okA) add(iA, qA);
+                if (okB) add(iB, qB);
             }
 #pragma unroll 1
             for (int i = 0; i < 10; i += 2) {
@@ -188,8 +200,8 @@ __global__ __launch_bounds__(COL_THREADS) void delays_column_kernel(Tables t, co
                 unsigned long long qA, qB;
                 prepare(ay, lane_value(a1, r), lane_value(z1.x, r), lane_value(z1.y, r), okA, iA, qA);
                 prepare(ay + 4, lane_value(a1, r + 4), lane_value(z1.x, r + 4), lane_value(z1.y, r + 4), okB, iB, qB);
-                if (okA) atomicAdd(&col[iA], qA);
-                if (okB) atomicAdd(&col[iB], qB);
+                if (okA) add(iA, qA);
+                if (okB) add(iB, qB);
             }
         }
     }
@@ -364,8 +376,7 @@ int group_lors(hrrt_delays &c) {
                 c.pairs_full += ring_pairs;
             }
     }
-    c.lds_bytes = (int)sizeof(unsigned long long) * c.nplanes;
-    if (c.lds_bytes + 1024 > 160 * 1024) return hrrt_dl::fail(HRRT_ERR_UNSUPPORTED, "sinogram column exceeds LDS");
+    if (c.nplanes > COLUMN_PLANES) return hrrt_dl::fail(HRRT_ERR_UNSUPPORTED, "more sinogram planes than the LDS column holds");
     if (c.nsino % 8) return hrrt_dl::fail(HRRT_ERR_UNSUPPORTED, "sinogram bins not a multiple of 8");
     return HRRT_OK;
 }
@@ -417,12 +428,6 @@ int hrrt_dl::upload(hrrt_delays &c) {
     DTRY(hipMemcpy(c.d_lor_off, count.data(), sizeof(int) * count.size(), hipMemcpyHostToDevice));
     DTRY(hipMemcpy(c.d_seg, seg.data(), sizeof(int2) * 128, hipMemcpyHostToDevice));
     DTRY(hipMemcpy(c.d_z, z.data(), sizeof(float2) * NY, hipMemcpyHostToDevice));
-    if (c.lds_bytes > 64 * 1024) {
-        DTRY(hipFuncSetAttribute((const void *)delays_column_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 c.lds_bytes));
-        DTRY(hipFuncSetAttribute((const void *)delays_column_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 c.lds_bytes));
-    }
     return HRRT_OK;
 }
 
@@ -558,13 +563,13 @@ extern "C" int hrrt_delays_smooth(hrrt_delays *c, const float *d_singles, float 
     sings_prepare_kernel<<<1, PREP_THREADS, 0, s>>>(d_singles, c->d_sings);
     DTRY(hipGetLastError());
     if (c->nplanes <= DIRECT_PLANES) {
-        delays_column_kernel<true><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, d_delays, 0);
+        delays_column_kernel<true><<<c->nsino, COL_THREADS, 0, s>>>(t, c->d_sings, tauftime, d_delays, 0);
         DTRY(hipGetLastError());
         return HRRT_OK;
     }
     const int cstride = (c->nplanes + 63) & ~63;
     if (!c->d_cols) DTRY(hipMalloc(&c->d_cols, sizeof(float) * (size_t)c->nsino * cstride));
-    delays_column_kernel<false><<<c->nsino, COL_THREADS, c->lds_bytes, s>>>(t, c->d_sings, tauftime, c->d_cols, cstride);
+    delays_column_kernel<false><<<c->nsino, COL_THREADS, 0, s>>>(t, c->d_sings, tauftime, c->d_cols, cstride);
     DTRY(hipGetLastError());
     column_transpose_kernel<<<dim3((c->nplanes + 64 * TT - 1) / (64 * TT), (c->nsino + 63) / 64), 256, 0, s>>>(c->d_cols, d_delays,
                                                                                                c->nsino, c->nplanes, cstride);
