@@ -186,3 +186,33 @@ def test_short_and_ragged_streams(torch_dev, stream, nwords):
         st = _check(torch_dev, g, D, words, d_words, mode=mode)
         if nwords == 0:
             assert st["events"] == 0
+
+
+def test_hot_bin_wraps_and_saturates(torch_dev, stream):
+    """One LOR with 70000 prompts and 300 delayed events, inside the random stream: int16 bins
+    wrap (trues 70000 - 300, prompts 70000), delayed bytes saturate at 255 (300 as int16
+    halves), and every mode, map and count matches the oracle."""
+    from hrrt_reconstruction_amd import lmhist
+    from hrrt_reconstruction_amd.phantom import lm_event_words
+    from hrrt_reconstruction_amd.randoms import GenDelays
+    print("lmhist hot bin", flush=True)
+    g, D = GenDelays(span=9), O.Delays(span=9)
+    spec = None
+    for ax in range(20, 50):
+        if O.lm_rebin(D, 3, 0, ax, 50, 0, 71 - ax, 52) >= 0:
+            spec, addr = (3, ax, 50, 71 - ax, 52, 0, 0), O.lm_rebin(D, 3, 0, ax, 50, 0, 71 - ax, 52)
+            break
+    assert spec is not None
+
+    def events(n, prompt):
+        cols = [np.full(n, v) for v in spec]
+        ew1, ew2 = lm_event_words(*cols, np.full(n, prompt))
+        return np.stack([ew1, ew2], axis=1).reshape(-1).astype(np.uint32)
+
+    cut = 2 * 300_000            # an event boundary inside the stream, past its first time tags
+    words = np.concatenate([stream[:cut], events(70000, True), events(300, False), stream[cut:]])
+    d_words = torch_dev.from_numpy(words.view(np.int32)).cuda()
+    for mode, kw in ((0, {}), (1, {}), (1, {"delayed_bytes": False}), (2, {})):
+        _check(torch_dev, g, D, words, d_words, mode=mode, **kw)
+    sino, delayed, _, _, _ = lmhist.histogram(g, d_words, mode=1)
+    assert int(delayed.reshape(-1)[addr]) == 255 and int(sino.reshape(-1)[addr]) == 70000 - 65536
