@@ -412,9 +412,10 @@ def listmode_leg(torch, events, steps, seed, cpu_events, span):
     return res
 
 
-def batched_frames(torch, nframes, iterations, seed):
+def batched_frames(torch, nframes, iterations, seed, tune=True):
     """Dynamic study in batched mode: every frame's subsets run in one launch per kernel, each subset
-    of an iteration replayed from a HIP graph.  FP32 form and FP16-input precision form."""
+    of an iteration replayed from a HIP graph.  FP32 form and FP16-input precision form, each with
+    the forward launch shape the batched-path autotuner picks (unless tune is False)."""
     from hrrt_reconstruction_amd import phantom as ph
     from hrrt_reconstruction_amd.osem import Osem3D
     g = Osem3D("cuda:0", **DYN_GEOM)
@@ -460,11 +461,18 @@ def batched_frames(torch, nframes, iterations, seed):
     nf16 = g.calculate_normfac_mixed(inv16)
     adds16 = g.to_half(adds)
     stream = torch.cuda.Stream()
-    res, out = {}, {}
+    res, out, shapes = {}, {}, {}
     init = g.init_image()
     for form in ("fp32", "fp16"):
         imgs = torch.stack([init for _ in range(nframes)])
         ll = torch.zeros((g.subsets, nframes), dtype=torch.float64, device="cuda")
+        if tune:
+            from hrrt_reconstruction_amd.autotune import tune_frames
+            shape, times = (tune_frames(g, imgs, nf32, prompts, adds) if form == "fp32"
+                            else tune_frames(g, imgs, nf16, prompts, adds16, mixed=True))
+            shapes[form] = list(shape)
+            log(f"batched {form} autotune forward " + ", ".join(f"{k}: {v:.3f} ms" for k, v in times.items())
+                + f" -> {shape}")
 
         def iterate(n):
             stream.wait_stream(torch.cuda.current_stream())   # inputs written on the current stream
@@ -506,10 +514,11 @@ def batched_frames(torch, nframes, iterations, seed):
         "fp16_batched_vs_fp32_single": round(out["single"] * nframes / out["fp16"], 3),
         "fp16_image_rel_l2_vs_fp32_max": float(f"{max(rel):.3e}"),
         "fp16_loglik_last_subset": [float(x) for x in res["fp16"][2][-1].cpu().numpy()[:3]],
+        "fp_launch_shape": shapes or None,
     }
 
 
-def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
+def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames, tune=True):
     """Combined form at full scale: a span-3 256x256x207 dynamic study of nframes frames (counts
     log-uniform between 1M and 300M, spheres washing out, randoms 10 %, scatter 15 %), FP16
     additive terms and sensitivity weights, int16 prompts, FP32 images; every subset of an
@@ -578,6 +587,12 @@ def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
     # FP16 batched form, all frames: once to capture the graphs, then timed on the same buffers
     init = g.init_image()
     imgs = torch.stack([init for _ in range(nframes)])
+    shapes = {}
+    if tune:
+        from hrrt_reconstruction_amd.autotune import tune_frames
+        shape, times = tune_frames(g, imgs, nf16, prompts, adds16, mixed=True)
+        shapes["fp16"] = list(shape)
+        log("c5: fp16 autotune forward " + ", ".join(f"{k}: {v:.3f} ms" for k, v in times.items()) + f" -> {shape}")
     run_batched(imgs, True, prompts, adds16, 1)
     imgs.copy_(init.expand_as(imgs))
     torch.cuda.synchronize()
@@ -594,6 +609,10 @@ def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
         pr32 = torch.stack([prompts[f] for f in sel])
         ad32 = torch.stack([adds32[f] for f in sel])
         i32 = torch.stack([init for _ in sel])
+        if tune:
+            shape, times = tune_frames(g, i32, nf32, pr32, ad32)
+            shapes["fp32"] = list(shape)
+            log("c5: fp32 autotune forward " + ", ".join(f"{k}: {v:.3f} ms" for k, v in times.items()) + f" -> {shape}")
         run_batched(i32, False, pr32, ad32, 1)
         i32.copy_(init.expand_as(i32))
         torch.cuda.synchronize()
@@ -647,6 +666,7 @@ def c5_leg(torch, nframes, iterations, seed, fp64_frames, fp32_frames):
                     "from HIP graphs for all frames",
         "frames": nframes, "iterations": iterations,
         "counts_m": [round(c / 1e6, 2) for c in counts],
+        "fp_launch_shape": shapes or None,
     })
     del prompts, adds16, adds32, imgs, nf16, nf32
     g.close()
@@ -836,7 +856,7 @@ def main():
                                 20_000_000 if args.cpu_subsets > 0 else 0, GEOM["span"])
     batched = None
     if args.frames > 0:
-        batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7)
+        batched = batched_frames(torch, args.frames, args.frame_iterations, args.seed + 7, tune=not args.no_tune)
     c1 = None
     if args.c1_iterations > 0:
         c1 = c1_leg(torch, args.seed + 19, args.c1_iterations, args.cpu_threads, args.cpu_subsets > 0)
@@ -847,7 +867,7 @@ def main():
         g.close()
         torch.cuda.empty_cache()
         c5 = c5_leg(torch, args.c5_frames, args.c5_iterations, args.seed + 17,
-                    args.c5_fp64_frames if args.cpu_subsets > 0 else 0, args.c5_fp32_frames)
+                    args.c5_fp64_frames if args.cpu_subsets > 0 else 0, args.c5_fp32_frames, tune=not args.no_tune)
 
     out = {
         "metric": "OSEM subset-iters/s span3 256²×207; batched frames/s; % HBM roofline",

@@ -11,7 +11,8 @@ incremental and the direct back projector is not a shape: the two differ by
 float rounding, as they do in the reference, and that choice stays with the
 caller (hrrt_set_backproject_schedule, the reference's -T).  The tuner times the
 candidates on the caller's own subset update with HIP events and keeps the
-fastest.
+fastest; tune_frames does the same on the batched update of every frame of a
+dynamic study, whose forward launches have a frame dimension of their own.
 """
 from __future__ import annotations
 
@@ -79,6 +80,43 @@ def tune_forward(g, image, normfac, prompt, add, subsets=(0, 8), repeats=2, cand
         for _ in range(repeats):
             for s in subsets:
                 g.osem_subset(work, normfac, prompt, add, s)
+        kt = g.timing_read()
+        g.timing_enable(False)
+        ms, n = kt["fp_ratio"]
+        results[shape] = ms / max(n, 1)
+    best = min(results, key=results.get)
+    g.set_launch_shape(*best)
+    del work
+    return best, results
+
+
+def tune_frames(g, images, normfac, prompts, adds, mixed=False, subsets=(0, 8), repeats=2, candidates=CANDIDATES):
+    """Forward launch shape for the batched path (hrrt_osem_subset_frames, or the FP16-input
+    form with mixed=True): every candidate timed on one subset update of all frames, the
+    fastest kept on the context.  Shapes give bit-identical results; a new shape drops the
+    context's captured iteration graphs, which the next iteration call captures again.
+    Returns ((tasks_per_thread, workgroups_per_ray), {shape: ms per forward launch})."""
+    import torch
+    work = images.clone()
+    results = {}
+
+    def run(s):
+        if mixed:
+            g.osem_subset_frames_mixed(work, normfac, prompts, adds, s)
+        else:
+            g.osem_subset_frames(work, normfac, prompts, adds, s)
+
+    for shape in candidates:
+        try:
+            g.set_launch_shape(*shape)
+        except Exception:
+            continue
+        run(subsets[0])                       # warm-up / first launch of this variant
+        torch.cuda.synchronize()
+        g.timing_enable(True)
+        for _ in range(repeats):
+            for s in subsets:
+                run(s)
         kt = g.timing_read()
         g.timing_enable(False)
         ms, n = kt["fp_ratio"]

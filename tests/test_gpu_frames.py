@@ -106,3 +106,50 @@ def test_frames_iteration_refuses_default_stream(torch_dev, oracle_lib):
     rc = _lib.lib().hrrt_osem_iteration_frames(g.ctx, 1, C.c_void_p(d.data_ptr()), C.c_void_p(nf.data_ptr()),
                                                C.c_void_p(pr.data_ptr()), C.c_void_p(ad.data_ptr()), None)
     assert rc == _lib.HRRT_ERR_ARG
+
+
+def test_frames_every_autotuner_shape(torch_dev, oracle_lib):
+    """The batched path under every forward launch shape the autotuner may pick equals the
+    default shape bit for bit, FP32 and FP16-input forms; tune_frames leaves one of them set,
+    and an iteration replayed after the change matches one run eagerly subset by subset."""
+    from hrrt_reconstruction_amd.autotune import CANDIDATES, tune_frames
+    from hrrt_reconstruction_amd.osem import Osem3D
+    torch = torch_dev
+    print("frames autotuner shapes", flush=True)
+    o = oracle_lib.Oracle(**GEOMS["tiny"])
+    g = Osem3D(**GEOMS["tiny"])
+    prompts, adds, nf = _frames(g, o, 3, 43)
+    d_nf = to_dev(nf)
+    img0 = o.init_image()
+    d_pr = to_dev(np.stack([g.sino_to_paircol(p) for p in prompts]))
+    d_add = to_dev(np.stack([g.sino_to_paircol(a) for a in adds]))
+    d_add16 = g.to_half(d_add)
+
+    def run(shape, mixed):
+        if shape is not None:
+            g.set_launch_shape(*shape)
+        d = to_dev(np.stack([img0] * 3))
+        for s in (0, 8):
+            if mixed:
+                g.osem_subset_frames_mixed(d, d_nf, d_pr, d_add16, s)
+            else:
+                g.osem_subset_frames(d, d_nf, d_pr, d_add, s)
+        return from_dev(d)
+
+    for mixed in (False, True):
+        ref = run((4, 0), mixed)
+        for shape in CANDIDATES:
+            assert np.array_equal(run(shape, mixed), ref), (shape, mixed)
+    best, times = tune_frames(g, to_dev(np.stack([img0] * 3)), d_nf, d_pr, d_add, subsets=(0,), repeats=1)
+    assert tuple(best) in CANDIDATES and len(times) == len(CANDIDATES)
+    stream = torch.cuda.Stream()
+    d = to_dev(np.stack([img0] * 3))
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        g.osem_iteration_frames(d, d_nf, d_pr, d_add)
+    stream.synchronize()
+    got = from_dev(d)
+    for f in range(3):
+        e = to_dev(img0)
+        g.calculate_osem3d(e, d_nf, d_pr[f], d_add[f], iterations=1)
+        assert np.array_equal(got[f], from_dev(e)), f"frame {f} after the tuned shape"
