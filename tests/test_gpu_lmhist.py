@@ -216,3 +216,32 @@ def test_hot_bin_wraps_and_saturates(torch_dev, stream):
         _check(torch_dev, g, D, words, d_words, mode=mode, **kw)
     sino, delayed, _, _, _ = lmhist.histogram(g, d_words, mode=1)
     assert int(delayed.reshape(-1)[addr]) == 255 and int(sino.reshape(-1)[addr]) == 70000 - 65536
+
+
+@pytest.mark.parametrize("which", ["prompt", "delayed"])
+def test_one_coincidence_map(torch_dev, stream, which):
+    """Only one of the two coincidence maps asked for (the other pointer NULL): that map equals
+    the oracle's, and the sinogram and counters are unchanged."""
+    import ctypes as C
+    from hrrt_reconstruction_amd import _lib, lmhist
+    from hrrt_reconstruction_amd.randoms import CRYSTALS, GenDelays
+    print(f"lmhist one coincidence map: {which}", flush=True)
+    g, D = GenDelays(span=9), O.Delays(span=9)
+    d_words = torch_dev.from_numpy(stream.view(np.int32)).cuda()
+    ref = O.lm_histogram(D, stream, mode=1)
+    n = g.nplanes * g.nviews * g.nprojs
+    sino = torch_dev.empty(n, dtype=torch_dev.int16, device="cuda")
+    delayed = torch_dev.empty(n, dtype=torch_dev.uint8, device="cuda")
+    cmap = torch_dev.full((CRYSTALS,), 7, dtype=torch_dev.int32, device="cuda")
+    opt = _lib.LmOptions(1, 0, 0, 0, 0)
+    st = _lib.LmStats()
+    p = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+    cp, cd = (cmap, None) if which == "prompt" else (None, cmap)
+    _lib.call("hrrt_lm_histogram", g._h, p(d_words), d_words.numel(), C.byref(opt), p(sino), p(delayed),
+              p(cp), p(cd), C.byref(st), C.c_void_p(torch_dev.cuda.current_stream().cuda_stream))
+    torch_dev.cuda.synchronize()
+    assert np.array_equal(sino.cpu().numpy(), ref["sino"].reshape(-1))
+    assert np.array_equal(delayed.cpu().numpy(), ref["delayed"].reshape(-1))
+    want = ref["coin_p"] if which == "prompt" else ref["coin_d"]
+    assert np.array_equal(cmap.cpu().numpy().view(np.uint32), want)
+    assert (st.prompts, st.randoms) == (int(ref["stats"][0]), int(ref["stats"][1]))
