@@ -680,7 +680,7 @@ def main():
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-subsets", type=int, default=4, help="timed subsets of the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-subsets", type=int, default=14, help="timed subsets of the CPU baseline, about 10 s on 8 host threads (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-tune", action="store_true", help="skip the forward-projector launch-shape autotuner")
     ap.add_argument("--frames", type=int, default=24, help="frames of the batched dynamic study (0 = skip)")
